@@ -1,0 +1,1053 @@
+// api.hip -- extern "C" entry points of libfastdose_hip (declared in include/fastdose_hip.h).
+//
+// Host orchestration of the CCCS beamlet-dose path: context set-up (density_h2d, bind_spectrum,
+// bind_kernel), the three stages (TermaComputeCollective, DoseComputeCollective,
+// BEV2PVCSInterp + addMat) and the dose-matrix ensemble (MatCSREnsemble, DoseMatConstruction's
+// per-beam iteration).  Reference: IMRTOpt/src/IMRTDoseMat.cu:14-301,
+// IMRTOpt/src/IMRTSparseMatEns.cu:10-336, FastDoseLib/src/{density,spectrum,kernel}.cu.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+#include <sys/stat.h>
+
+#include "fd_internal.h"
+#include "geometry.h"
+#include "kernels.h"
+
+namespace fdi {
+
+namespace {
+thread_local std::string g_error;
+}
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+namespace {
+
+#define FD_HIP(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            set_error(std::string(#call) + ": " + hipGetErrorString(e_));                   \
+            return FD_ERR_HIP;                                                              \
+        }                                                                                   \
+    } while (0)
+
+#define FD_TRY(call)              \
+    do {                                  \
+        int r_ = (call);                  \
+        if (r_ != FD_OK) return r_;       \
+    } while (0)
+
+// Growable device allocation.
+template <typename T>
+struct DevBuf {
+    T* ptr = nullptr;
+    size_t cap = 0;  // elements
+    int reserve(size_t n) {
+        if (n <= cap) return FD_OK;
+        size_t want = std::max(n, cap + cap / 2);
+        if (ptr) {
+            hipError_t e = hipFree(ptr);
+            ptr = nullptr;
+            cap = 0;
+            if (e != hipSuccess) {
+                set_error(std::string("hipFree: ") + hipGetErrorString(e));
+                return FD_ERR_HIP;
+            }
+        }
+        hipError_t e = hipMalloc((void**)&ptr, want * sizeof(T));
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+            return FD_ERR_HIP;
+        }
+        cap = want;
+        return FD_OK;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+};
+
+// Growable device buffer that keeps its contents (used by the dose-matrix ensemble; doubles
+// like MatCSREnsemble::addMat, IMRTSparseMatEns.cu:125-149).
+template <typename T>
+int grow_keep(DevBuf<T>& b, size_t used, size_t need, hipStream_t stream) {
+    if (need <= b.cap) return FD_OK;
+    size_t want = std::max<size_t>(need, b.cap ? b.cap * 2 : 1024);
+    T* np = nullptr;
+    FD_HIP(hipMalloc((void**)&np, want * sizeof(T)));
+    if (used && b.ptr) FD_HIP(hipMemcpyAsync(np, b.ptr, used * sizeof(T), hipMemcpyDeviceToDevice, stream));
+    FD_HIP(hipStreamSynchronize(stream));
+    if (b.ptr) FD_HIP(hipFree(b.ptr));
+    b.ptr = np;
+    b.cap = want;
+    return FD_OK;
+}
+
+}  // namespace
+
+}  // namespace fdi
+
+using namespace fdi;
+
+struct fd_ctx {
+    int device = 0;
+    fd_volume vol{};
+    float* density = nullptr;
+    fd_spectrum spectrum{};
+    fd_kernel kernel{};
+    SpectrumDev spec_dev{};
+    ConeTable cones{};
+    ConeTable* cones_dev = nullptr;
+    int n_fwd = 0;
+    int options = 0;
+
+    // per-call staging of beamlet descriptors (pinned) and scratch
+    BeamletDev* stage = nullptr;
+    size_t stage_cap = 0;
+    hipEvent_t stage_done = nullptr;
+    DevBuf<BeamletDev> desc;
+    DevBuf<float> partial;
+    DevBuf<float> fluence, terma, dens_bev, dose_bev;
+    DevBuf<uint32_t> cand_count, nnz_count;
+    DevBuf<uint64_t> cand_off, nnz_off;
+    DevBuf<uint64_t> tmp_cols;
+    DevBuf<float> tmp_vals;
+    uint64_t* host_totals = nullptr;  // pinned, 2 entries
+    // kernel timers: slot k uses ev[2k] / ev[2k+1]
+    hipEvent_t ev[2 * FD_T_COUNT] = {};
+    bool ev_used[FD_T_COUNT] = {};
+    bool timing() const { return (options & FD_OPT_TIMING) != 0 && ev[0] != nullptr; }
+    void mark(int slot, int which, hipStream_t s) {
+        if (!timing()) return;
+        (void)hipEventRecord(ev[2 * slot + which], s);
+        if (which == 1) ev_used[slot] = true;
+    }
+};
+
+struct fd_dosemat {
+    uint64_t n_cols = 0;
+    std::vector<uint64_t> rows;       // numRowsPerMat
+    std::vector<uint64_t> nnz;        // NonZeroElements
+    std::vector<uint64_t> nnz_off;    // start of each matrix in columns/values
+    std::vector<uint64_t> off_off;    // OffsetBufferIdx
+    DevBuf<uint64_t> offsets;         // offsetsBuffer
+    DevBuf<uint64_t> columns;         // columnsBuffer
+    DevBuf<float> values;             // valuesBuffer
+    uint64_t used_off = 0, used_nnz = 0;
+};
+
+namespace {
+
+int check_ctx(fd_ctx* ctx) {
+    if (!ctx) {
+        set_error("null context");
+        return FD_ERR_ARG;
+    }
+    FD_HIP(hipSetDevice(ctx->device));
+    return FD_OK;
+}
+
+int check_beams(const fd_beam* beams, int n) {
+    if (n < 0 || (n > 0 && !beams)) {
+        set_error("invalid beamlet array");
+        return FD_ERR_ARG;
+    }
+    for (int i = 0; i < n; i++) {
+        if (beams[i].fmap_size[0] != (uint32_t)kFmap || beams[i].fmap_size[1] != (uint32_t)kFmap) {
+            set_error("the HIP kernels implement 16 x 16 BEV grids (subFluenceDim 16)");
+            return FD_ERR_UNSUPPORTED;
+        }
+        if (beams[i].long_dim == 0 || beams[i].long_dim > (1u << 20)) {
+            set_error("beamlet long_dim out of range (run fd_calc_range first)");
+            return FD_ERR_ARG;
+        }
+        if (!(beams[i].sad > 0.0f) || !(beams[i].long_spacing > 0.0f) || !(beams[i].beamlet_size[0] > 0.0f) ||
+            !(beams[i].beamlet_size[1] > 0.0f)) {
+            set_error("beamlet sad, long_spacing and beamlet_size must be positive");
+            return FD_ERR_ARG;
+        }
+    }
+    return FD_OK;
+}
+
+int check_stride(const fd_beam* beams, int n, uint64_t stride) {
+    for (int i = 0; i < n; i++)
+        if ((uint64_t)beams[i].long_dim * kNpix > stride) {
+            set_error("stride smaller than long_dim * 256 of a beamlet");
+            return FD_ERR_ARG;
+        }
+    return FD_OK;
+}
+
+// Host descriptors -> device, through pinned staging.
+int upload_beams(fd_ctx* ctx, const fd_beam* beams, int n, uint64_t stride, hipStream_t stream) {
+    if (n == 0) return FD_OK;
+    if ((size_t)n > ctx->stage_cap) {
+        if (ctx->stage) {
+            FD_HIP(hipEventSynchronize(ctx->stage_done));
+            FD_HIP(hipHostFree(ctx->stage));
+            ctx->stage = nullptr;
+            ctx->stage_cap = 0;
+        }
+        FD_HIP(hipHostMalloc((void**)&ctx->stage, (size_t)n * sizeof(BeamletDev), hipHostMallocDefault));
+        ctx->stage_cap = n;
+    } else {
+        FD_HIP(hipEventSynchronize(ctx->stage_done));  // previous upload has left the staging area
+    }
+    for (int i = 0; i < n; i++) fill_beamlet_dev(beams[i], (uint64_t)i * stride, ctx->stage[i]);
+    FD_TRY(ctx->desc.reserve(n));
+    FD_HIP(hipMemcpyAsync(ctx->desc.ptr, ctx->stage, (size_t)n * sizeof(BeamletDev), hipMemcpyHostToDevice, stream));
+    FD_HIP(hipEventRecord(ctx->stage_done, stream));
+    return FD_OK;
+}
+
+int max_long_dim(const fd_beam* beams, int n) {
+    int m = 0;
+    for (int i = 0; i < n; i++) m = std::max(m, (int)beams[i].long_dim);
+    return m;
+}
+
+int run_terma(fd_ctx* ctx, int n, const float* fluence, float* terma, float* dens, hipStream_t stream) {
+    ctx->mark(FD_T_TERMA, 0, stream);
+    hipError_t e = launch_terma(ctx->desc.ptr, n, fluence, ctx->density, ctx->vol, ctx->spec_dev,
+                                (ctx->options & FD_OPT_RPL_FP64) != 0, terma, dens, stream);
+    ctx->mark(FD_T_TERMA, 1, stream);
+    if (e != hipSuccess) {
+        set_error(std::string("terma launch: ") + hipGetErrorString(e));
+        return FD_ERR_HIP;
+    }
+    return FD_OK;
+}
+
+int run_cccs(fd_ctx* ctx, int n, int max_L, const float* terma, const float* dens, float* dose, uint64_t stride,
+             hipStream_t stream) {
+    FD_TRY(ctx->partial.reserve((size_t)n * std::max(ctx->n_fwd, 1) * stride));
+    hipEvent_t evs[3];
+    const bool t = ctx->timing();
+    if (t) {
+        evs[0] = ctx->ev[2 * FD_T_CCCS];
+        evs[1] = ctx->ev[2 * FD_T_CCCS + 1];
+        evs[2] = ctx->ev[2 * FD_T_CONE_SUM + 1];
+    }
+    hipError_t e = launch_cccs(ctx->desc.ptr, n, max_L, ctx->cones_dev, ctx->n_fwd, ctx->cones.n_phi,
+                               (ctx->options & FD_OPT_EXACT_G) != 0, terma, dens, ctx->partial.ptr, dose, stride,
+                               stream, t ? evs : nullptr);
+    if (t) {
+        // the cone sum starts where the CCCS kernel stops (fd_ctx_timings pairs them)
+        ctx->ev_used[FD_T_CCCS] = true;
+        ctx->ev_used[FD_T_CONE_SUM] = true;
+    }
+    if (e != hipSuccess) {
+        set_error(std::string("cccs launch: ") + hipGetErrorString(e));
+        return FD_ERR_HIP;
+    }
+    return FD_OK;
+}
+
+// Stages 1-2 of the resampler; leaves the candidate/nonzero offsets in ctx and returns the
+// nonzero total (synchronises the stream twice to size the buffers).
+int run_resample_values(fd_ctx* ctx, int n, const float* dose, float extent, uint64_t* nnz_total,
+                        hipStream_t stream) {
+    const size_t Z = ctx->vol.dim[2];
+    const size_t segs = (size_t)n * Z;
+    FD_TRY(ctx->cand_count.reserve(segs));
+    FD_TRY(ctx->nnz_count.reserve(segs));
+    FD_TRY(ctx->cand_off.reserve(segs + 1));
+    FD_TRY(ctx->nnz_off.reserve(segs + 1));
+    ctx->mark(FD_T_RESAMPLE_COUNT, 0, stream);
+    hipError_t e = launch_resample_count(ctx->desc.ptr, n, ctx->vol, ctx->cand_count.ptr, stream);
+    if (e == hipSuccess) e = launch_scan_u32(ctx->cand_count.ptr, segs, ctx->cand_off.ptr, ctx->cand_off.ptr + segs,
+                                             stream);
+    ctx->mark(FD_T_RESAMPLE_COUNT, 1, stream);
+    if (e != hipSuccess) {
+        set_error(std::string("resample count: ") + hipGetErrorString(e));
+        return FD_ERR_HIP;
+    }
+    FD_HIP(hipMemcpyAsync(ctx->host_totals, ctx->cand_off.ptr + segs, sizeof(uint64_t), hipMemcpyDeviceToHost,
+                          stream));
+    FD_HIP(hipStreamSynchronize(stream));
+    const uint64_t ncand = ctx->host_totals[0];
+    FD_TRY(ctx->tmp_cols.reserve(std::max<uint64_t>(ncand, 1)));
+    FD_TRY(ctx->tmp_vals.reserve(std::max<uint64_t>(ncand, 1)));
+    ctx->mark(FD_T_RESAMPLE_VALUES, 0, stream);
+    e = launch_resample_values(ctx->desc.ptr, n, ctx->vol, extent, dose, ctx->cand_off.ptr, ctx->tmp_cols.ptr,
+                               ctx->tmp_vals.ptr, ctx->nnz_count.ptr, stream);
+    if (e == hipSuccess) e = launch_scan_u32(ctx->nnz_count.ptr, segs, ctx->nnz_off.ptr, ctx->nnz_off.ptr + segs,
+                                             stream);
+    ctx->mark(FD_T_RESAMPLE_VALUES, 1, stream);
+    if (e != hipSuccess) {
+        set_error(std::string("resample values: ") + hipGetErrorString(e));
+        return FD_ERR_HIP;
+    }
+    FD_HIP(hipMemcpyAsync(ctx->host_totals + 1, ctx->nnz_off.ptr + segs, sizeof(uint64_t), hipMemcpyDeviceToHost,
+                          stream));
+    FD_HIP(hipStreamSynchronize(stream));
+    *nnz_total = ctx->host_totals[1];
+    return FD_OK;
+}
+
+int run_resample_compact(fd_ctx* ctx, int n, uint64_t* offsets, uint64_t* columns, float* values,
+                         hipStream_t stream) {
+    ctx->mark(FD_T_RESAMPLE_COMPACT, 0, stream);
+    hipError_t e = launch_resample_compact(n, (int)ctx->vol.dim[2], ctx->cand_off.ptr, ctx->nnz_off.ptr,
+                                           ctx->nnz_count.ptr, ctx->tmp_cols.ptr, ctx->tmp_vals.ptr, offsets,
+                                           columns, values, stream);
+    ctx->mark(FD_T_RESAMPLE_COMPACT, 1, stream);
+    if (e != hipSuccess) {
+        set_error(std::string("resample compact: ") + hipGetErrorString(e));
+        return FD_ERR_HIP;
+    }
+    return FD_OK;
+}
+
+ConeTable build_cones(const fd_kernel& k, int* n_fwd) {
+    ConeTable c;
+    std::memset(&c, 0, sizeof(c));
+    // forward cones (DoseComp.cu:265 skips theta_mid > pi/2), widest first for load balance
+    std::vector<int> fwd;
+    for (int i = 0; i < k.n_theta; i++)
+        if (!((double)k.theta_mid[i] > 1.57079632679489661923)) fwd.push_back(i);
+    std::stable_sort(fwd.begin(), fwd.end(), [&](int a, int b) { return k.theta_mid[a] > k.theta_mid[b]; });
+    c.n_fwd = (int)fwd.size();
+    c.n_phi = k.n_phi;
+    for (int t = 0; t < c.n_fwd; t++) {
+        const int i = fwd[t];
+        c.theta_idx[t] = i;
+        c.A_a[t] = k.A[i] / k.a[i];
+        c.B_b[t] = k.B[i] / k.b[i];
+        c.a[t] = k.a[i];
+        c.b[t] = k.b[i];
+        const double st = std::sin((double)k.theta_mid[i]), ct = std::cos((double)k.theta_mid[i]);
+        for (int f = 0; f < k.n_phi; f++) {
+            const double sp = std::sin((double)k.phi[f]), cp = std::cos((double)k.phi[f]);
+            c.ux[t][f] = (float)(st * cp);
+            c.uy[t][f] = (float)(st * sp);
+            c.uz[t][f] = (float)ct;
+        }
+    }
+    *n_fwd = c.n_fwd;
+    return c;
+}
+
+int append_matrix(fd_dosemat* m, uint64_t rows, uint64_t nnz, hipStream_t stream, uint64_t** off_dst,
+                  uint64_t** col_dst, float** val_dst) {
+    FD_TRY(grow_keep(m->offsets, m->used_off, m->used_off + rows + 1, stream));
+    FD_TRY(grow_keep(m->columns, m->used_nnz, m->used_nnz + std::max<uint64_t>(nnz, 1), stream));
+    FD_TRY(grow_keep(m->values, m->used_nnz, m->used_nnz + std::max<uint64_t>(nnz, 1), stream));
+    *off_dst = m->offsets.ptr + m->used_off;
+    *col_dst = m->columns.ptr + m->used_nnz;
+    *val_dst = m->values.ptr + m->used_nnz;
+    m->rows.push_back(rows);
+    m->nnz.push_back(nnz);
+    m->off_off.push_back(m->used_off);
+    m->nnz_off.push_back(m->used_nnz);
+    m->used_off += rows + 1;
+    m->used_nnz += nnz;
+    return FD_OK;
+}
+
+bool write_file(const std::string& path, const void* data, size_t bytes) {
+    std::ofstream f(path, std::ios::binary | std::ios::trunc);
+    if (!f.is_open()) return false;
+    if (bytes) f.write((const char*)data, (std::streamsize)bytes);
+    return (bool)f;
+}
+
+bool read_file(const std::string& path, std::vector<char>& out) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f.is_open()) return false;
+    f.seekg(0, std::ios::end);
+    std::streamoff n = f.tellg();
+    f.seekg(0, std::ios::beg);
+    out.resize((size_t)n);
+    if (n) f.read(out.data(), n);
+    return (bool)f;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fd_last_error(void) { return g_error.c_str(); }
+const char* fd_version(void) { return "fastdose_hip 0.1 (gfx950)"; }
+
+int fd_calc_range(fd_beam* beam, const fd_volume* vol) {
+    if (!beam || !vol) {
+        set_error("null argument");
+        return FD_ERR_ARG;
+    }
+    return calc_range(*beam, *vol);
+}
+
+int fd_beamlet_init(const fd_bundle* bundle, int ix, int iy, const fd_volume* vol, fd_beam* out,
+                    float* fluence_out) {
+    if (!bundle || !vol || !out) {
+        set_error("null argument");
+        return FD_ERR_ARG;
+    }
+    if (ix < 0 || iy < 0 || ix >= bundle->fluence_dim[0] || iy >= bundle->fluence_dim[1]) {
+        set_error("beamlet index outside the fluence map");
+        return FD_ERR_ARG;
+    }
+    std::memset(out, 0, sizeof(*out));
+    return beamlet_init(*bundle, ix, iy, *vol, *out, fluence_out);
+}
+
+int fd_beamlets_init(const fd_bundle* bundle, const uint8_t* flags, const fd_volume* vol, fd_beam* out,
+                     float* fluence_out, int* n_out) {
+    if (!bundle || !flags || !vol || !out || !n_out) {
+        set_error("null argument");
+        return FD_ERR_ARG;
+    }
+    const int fx = bundle->fluence_dim[0], fy = bundle->fluence_dim[1];
+    const size_t npix = (size_t)bundle->sub_dim[0] * bundle->sub_dim[1];
+    int count = 0;
+    for (int j = 0; j < fy; j++)
+        for (int i = 0; i < fx; i++) {
+            if (!flags[i + j * fx]) continue;
+            std::memset(&out[count], 0, sizeof(fd_beam));
+            int r = beamlet_init(*bundle, i, j, *vol, out[count], fluence_out ? fluence_out + npix * count : nullptr);
+            if (r != FD_OK) return r;
+            count++;
+        }
+    *n_out = count;
+    return FD_OK;
+}
+
+int fd_ptv_isocenter(const uint8_t* mask, const fd_volume* vol, float iso[3]) {
+    if (!mask || !vol || !iso) {
+        set_error("null argument");
+        return FD_ERR_ARG;
+    }
+    return ptv_isocenter(mask, *vol, iso);
+}
+
+int fd_beamlet_flags_host(const fd_bundle* bundle, const uint8_t* ptv_mask, const fd_volume* vol,
+                          uint8_t* flags) {
+    if (!bundle || !ptv_mask || !vol || !flags) {
+        set_error("null argument");
+        return FD_ERR_ARG;
+    }
+    return beamlet_flags(*bundle, ptv_mask, *vol, flags);
+}
+
+int fd_read_spectrum(const char* path, fd_spectrum* out) {
+    if (!path || !out) {
+        set_error("null argument");
+        return FD_ERR_ARG;
+    }
+    std::ifstream f(path);
+    if (!f.is_open()) {
+        set_error(std::string("cannot open spectrum file ") + path);
+        return FD_ERR_IO;
+    }
+    std::memset(out, 0, sizeof(*out));
+    std::string line;
+    float total = 0.0f;
+    int n = 0;
+    while (std::getline(f, line)) {
+        if (line.find_first_not_of(" \t\r\n") == std::string::npos) continue;  // blank rows
+        if (n >= FD_MAX_TABLE) {
+            set_error("more than 24 spectrum rows");
+            return FD_ERR_ARG;
+        }
+        std::istringstream row(line);
+        std::string label;
+        row >> out->energy[n] >> out->fluence[n] >> out->mu[n] >> out->mu_en[n] >> label;
+        if (row.fail() && !row.eof()) {
+            set_error("malformed spectrum row: " + line);
+            return FD_ERR_IO;
+        }
+        total += out->fluence[n];
+        n++;
+    }
+    out->n = n;
+    for (int i = 0; i < n; i++) out->fluence[i] /= total;
+    return FD_OK;
+}
+
+int fd_read_kernel(const char* path, int n_phi, fd_kernel* out) {
+    if (!path || !out) {
+        set_error("null argument");
+        return FD_ERR_ARG;
+    }
+    if (n_phi <= 0 || n_phi > kMaxPhi) {
+        set_error("n_phi must be in [1, 16]");
+        return FD_ERR_ARG;
+    }
+    std::ifstream f(path);
+    if (!f.is_open()) {
+        set_error(std::string("cannot open kernel file ") + path);
+        return FD_ERR_IO;
+    }
+    std::memset(out, 0, sizeof(*out));
+    std::string line;
+    std::getline(f, line);  // header
+    int n = 0;
+    while (std::getline(f, line)) {
+        if (line.find_first_not_of(" \t\r\n") == std::string::npos) continue;
+        if (n >= FD_MAX_TABLE) {
+            set_error("more than 24 kernel rows");
+            return FD_ERR_ARG;
+        }
+        std::istringstream row(line);
+        row >> out->theta_begin[n] >> out->theta_end[n] >> out->A[n] >> out->a[n] >> out->B[n] >> out->b[n];
+        if (row.fail()) {
+            set_error("malformed kernel row: " + line);
+            return FD_ERR_IO;
+        }
+        out->theta_mid[n] = (out->theta_begin[n] + out->theta_end[n]) / 2;
+        n++;
+    }
+    out->n_theta = n;
+    out->n_phi = n_phi;
+    const float step = 2 * 3.141592654f / n_phi;
+    for (int i = 0; i < n_phi; i++) out->phi[i] = (float)((i + 0.5) * step);
+    return FD_OK;
+}
+
+int fd_ctx_create(int device, const fd_volume* vol, const float* density_host, const fd_spectrum* spectrum,
+                  const fd_kernel* kernel, fd_ctx** out) {
+    if (!vol || !density_host || !spectrum || !kernel || !out) {
+        set_error("null argument");
+        return FD_ERR_ARG;
+    }
+    *out = nullptr;
+    if (spectrum->n <= 0 || spectrum->n > FD_MAX_TABLE) {
+        set_error("spectrum must have 1..24 bins");
+        return FD_ERR_ARG;
+    }
+    if (kernel->n_theta <= 0 || kernel->n_theta > FD_MAX_TABLE || kernel->n_phi <= 0 || kernel->n_phi > kMaxPhi) {
+        set_error("kernel must have 1..24 zenith and 1..16 azimuth angles");
+        return FD_ERR_ARG;
+    }
+    for (int d = 0; d < 3; d++)
+        if (vol->dim[d] == 0 || !(vol->voxel_size[d] > 0.0f)) {
+            set_error("volume dims and voxel sizes must be positive");
+            return FD_ERR_ARG;
+        }
+    if (vol->dim[1] > 8192) {
+        set_error("volume y dimension above 8192 (resampler row table lives in LDS)");
+        return FD_ERR_UNSUPPORTED;
+    }
+    for (int i = 0; i < kernel->n_theta; i++) {
+        const double th = kernel->theta_mid[i];
+        if (!(th > 1.57079632679489661923) && std::cos(th) < kMinCosTheta) {
+            set_error("forward cone too close to 90 degrees (cos(theta_mid) < 0.01)");
+            return FD_ERR_UNSUPPORTED;
+        }
+    }
+    int ndev = 0;
+    FD_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) {
+        set_error("device index out of range");
+        return FD_ERR_ARG;
+    }
+    FD_HIP(hipSetDevice(device));
+    fd_ctx* c = new fd_ctx();
+    c->device = device;
+    c->vol = *vol;
+    c->spectrum = *spectrum;
+    c->kernel = *kernel;
+    c->spec_dev.n = spectrum->n;
+    for (int e = 0; e < spectrum->n; e++) {
+        c->spec_dev.w[e] = spectrum->fluence[e] * spectrum->energy[e] * spectrum->mu[e];
+        c->spec_dev.mu[e] = spectrum->mu[e];
+    }
+    c->cones = build_cones(*kernel, &c->n_fwd);
+    const size_t nvox = (size_t)vol->dim[0] * vol->dim[1] * vol->dim[2];
+    auto fail = [&](hipError_t e, const char* what) {
+        set_error(std::string(what) + ": " + hipGetErrorString(e));
+        fd_ctx_destroy(c);
+        return FD_ERR_HIP;
+    };
+    hipError_t e;
+    if ((e = hipMalloc((void**)&c->density, nvox * sizeof(float))) != hipSuccess) return fail(e, "hipMalloc density");
+    if ((e = hipMemcpy(c->density, density_host, nvox * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(e, "density upload");
+    if ((e = hipMalloc((void**)&c->cones_dev, sizeof(ConeTable))) != hipSuccess) return fail(e, "hipMalloc cones");
+    if ((e = hipMemcpy(c->cones_dev, &c->cones, sizeof(ConeTable), hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(e, "cone upload");
+    if ((e = hipEventCreateWithFlags(&c->stage_done, hipEventDisableTiming)) != hipSuccess)
+        return fail(e, "hipEventCreate");
+    if ((e = hipHostMalloc((void**)&c->host_totals, 4 * sizeof(uint64_t), hipHostMallocDefault)) != hipSuccess)
+        return fail(e, "hipHostMalloc");
+    for (auto& ev : c->ev)
+        if ((e = hipEventCreate(&ev)) != hipSuccess) return fail(e, "hipEventCreate");
+    *out = c;
+    return FD_OK;
+}
+
+int fd_ctx_destroy(fd_ctx* ctx) {
+    if (!ctx) return FD_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    if (ctx->density) (void)hipFree(ctx->density);
+    if (ctx->cones_dev) (void)hipFree(ctx->cones_dev);
+    if (ctx->stage) (void)hipHostFree(ctx->stage);
+    if (ctx->host_totals) (void)hipHostFree(ctx->host_totals);
+    if (ctx->stage_done) (void)hipEventDestroy(ctx->stage_done);
+    for (auto& ev : ctx->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    ctx->desc.release();
+    ctx->partial.release();
+    ctx->fluence.release();
+    ctx->terma.release();
+    ctx->dens_bev.release();
+    ctx->dose_bev.release();
+    ctx->cand_count.release();
+    ctx->nnz_count.release();
+    ctx->cand_off.release();
+    ctx->nnz_off.release();
+    ctx->tmp_cols.release();
+    ctx->tmp_vals.release();
+    delete ctx;
+    return FD_OK;
+}
+
+int fd_ctx_set_options(fd_ctx* ctx, int options) {
+    if (!ctx) {
+        set_error("null context");
+        return FD_ERR_ARG;
+    }
+    if (options & ~(FD_OPT_EXACT_G | FD_OPT_RPL_FP64 | FD_OPT_TIMING)) {
+        set_error("unknown option bits");
+        return FD_ERR_ARG;
+    }
+    ctx->options = options;
+    return FD_OK;
+}
+
+int fd_ctx_timings(fd_ctx* ctx, float* ms, int n) {
+    if (!ctx || !ms || n < 0) {
+        set_error("null argument");
+        return FD_ERR_ARG;
+    }
+    for (int k = 0; k < n; k++) {
+        ms[k] = -1.0f;
+        if (k >= FD_T_COUNT || !ctx->ev_used[k] || !ctx->ev[0]) continue;
+        hipEvent_t a = (k == FD_T_CONE_SUM) ? ctx->ev[2 * FD_T_CCCS + 1] : ctx->ev[2 * k];
+        hipEvent_t b = ctx->ev[2 * k + 1];
+        FD_HIP(hipEventSynchronize(b));
+        float t = 0.0f;
+        FD_HIP(hipEventElapsedTime(&t, a, b));
+        ms[k] = t;
+    }
+    return FD_OK;
+}
+
+int fd_ctx_density(fd_ctx* ctx, const float** density_dev) {
+    if (!ctx || !density_dev) {
+        set_error("null argument");
+        return FD_ERR_ARG;
+    }
+    *density_dev = ctx->density;
+    return FD_OK;
+}
+
+int fd_terma(fd_ctx* ctx, const fd_beam* beams, int n, const float* fluence_dev, float* terma_dev,
+             float* density_bev_dev, uint64_t stride, void* stream) {
+    FD_TRY(check_ctx(ctx));
+    FD_TRY(check_beams(beams, n));
+    FD_TRY(check_stride(beams, n, stride));
+    if (n == 0) return FD_OK;
+    if (!fluence_dev || !terma_dev || !density_bev_dev) {
+        set_error("null device buffer");
+        return FD_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    FD_TRY(upload_beams(ctx, beams, n, stride, s));
+    return run_terma(ctx, n, fluence_dev, terma_dev, density_bev_dev, s);
+}
+
+int fd_cccs(fd_ctx* ctx, const fd_beam* beams, int n, const float* terma_dev, const float* density_bev_dev,
+            float* dose_bev_dev, uint64_t stride, void* stream) {
+    FD_TRY(check_ctx(ctx));
+    FD_TRY(check_beams(beams, n));
+    FD_TRY(check_stride(beams, n, stride));
+    if (n == 0) return FD_OK;
+    if (!terma_dev || !density_bev_dev || !dose_bev_dev) {
+        set_error("null device buffer");
+        return FD_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    FD_TRY(upload_beams(ctx, beams, n, stride, s));
+    return run_cccs(ctx, n, max_long_dim(beams, n), terma_dev, density_bev_dev, dose_bev_dev, stride, s);
+}
+
+int fd_resample_csr(fd_ctx* ctx, const fd_beam* beams, int n, const float* dose_bev_dev, uint64_t stride,
+                    float extent, fd_csr* out, void* stream) {
+    FD_TRY(check_ctx(ctx));
+    FD_TRY(check_beams(beams, n));
+    FD_TRY(check_stride(beams, n, stride));
+    if (!out || !out->offsets || (n > 0 && !dose_bev_dev)) {
+        set_error("null argument");
+        return FD_ERR_ARG;
+    }
+    if (!(extent >= 0.0f)) {
+        set_error("extent must be non-negative");
+        return FD_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    out->rows = (uint64_t)n;
+    if (n == 0) {
+        out->nnz = 0;
+        FD_HIP(hipMemsetAsync(out->offsets, 0, sizeof(uint64_t), s));
+        return FD_OK;
+    }
+    FD_TRY(upload_beams(ctx, beams, n, stride, s));
+    uint64_t nnz = 0;
+    FD_TRY(run_resample_values(ctx, n, dose_bev_dev, extent, &nnz, s));
+    out->nnz = nnz;
+    if (nnz > out->capacity || (nnz > 0 && (!out->columns || !out->values))) {
+        set_error("CSR capacity too small; nnz reported in out->nnz");
+        return FD_ERR_CAPACITY;
+    }
+    return run_resample_compact(ctx, n, out->offsets, out->columns, out->values, s);
+}
+
+int fd_dense_to_csr(const float* dense_dev, uint64_t rows, uint64_t cols, fd_csr* out, void* stream) {
+    if (!out || !out->offsets || (rows > 0 && cols > 0 && !dense_dev)) {
+        set_error("null argument");
+        return FD_ERR_ARG;
+    }
+    if (cols >= (1ull << 32) || rows >= (1ull << 31)) {
+        set_error("dense matrix too large for the per-row u32 counts");
+        return FD_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    out->rows = rows;
+    uint32_t* cnt = nullptr;
+    uint64_t* off = nullptr;
+    uint64_t* host = nullptr;
+    auto cleanup = [&]() {
+        if (cnt) (void)hipFree(cnt);
+        if (off) (void)hipFree(off);
+        if (host) (void)hipHostFree(host);
+    };
+    auto hip_fail = [&](hipError_t e, const char* what) {
+        set_error(std::string(what) + ": " + hipGetErrorString(e));
+        cleanup();
+        return FD_ERR_HIP;
+    };
+    hipError_t e;
+    if ((e = hipMalloc((void**)&cnt, std::max<uint64_t>(rows, 1) * sizeof(uint32_t))) != hipSuccess)
+        return hip_fail(e, "hipMalloc");
+    if ((e = hipMalloc((void**)&off, (rows + 1) * sizeof(uint64_t))) != hipSuccess) return hip_fail(e, "hipMalloc");
+    if ((e = hipHostMalloc((void**)&host, sizeof(uint64_t), hipHostMallocDefault)) != hipSuccess)
+        return hip_fail(e, "hipHostMalloc");
+    if (rows > 0 && cols > 0) {
+        if ((e = launch_dense_count(dense_dev, rows, cols, cnt, s)) != hipSuccess) return hip_fail(e, "count");
+    } else if (rows > 0) {
+        if ((e = hipMemsetAsync(cnt, 0, rows * sizeof(uint32_t), s)) != hipSuccess) return hip_fail(e, "memset");
+    }
+    if ((e = launch_scan_u32(cnt, rows, off, off + rows, s)) != hipSuccess) return hip_fail(e, "scan");
+    if ((e = hipMemcpyAsync(host, off + rows, sizeof(uint64_t), hipMemcpyDeviceToHost, s)) != hipSuccess)
+        return hip_fail(e, "copy");
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "sync");
+    out->nnz = host[0];
+    if (out->nnz > out->capacity || (out->nnz > 0 && (!out->columns || !out->values))) {
+        cleanup();
+        set_error("CSR capacity too small; nnz reported in out->nnz");
+        return FD_ERR_CAPACITY;
+    }
+    if (rows > 0 && cols > 0 && out->nnz > 0)
+        if ((e = launch_dense_fill(dense_dev, rows, cols, off, out->columns, out->values, s)) != hipSuccess)
+            return hip_fail(e, "fill");
+    if ((e = launch_write_offsets(off, off + rows, rows, out->offsets, s)) != hipSuccess) return hip_fail(e, "offsets");
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "sync");
+    cleanup();
+    return FD_OK;
+}
+
+int fd_cccs_segments(fd_ctx* ctx, const fd_beam* beams, int n, int64_t* nseg_out) {
+    if (!ctx || !nseg_out) {
+        set_error("null argument");
+        return FD_ERR_ARG;
+    }
+    FD_TRY(check_beams(beams, n));
+    // The same FP32 lattice walk as the kernel, without the physics.
+    for (int i = 0; i < n; i++) {
+        const fd_beam& B = beams[i];
+        int64_t total = 0;
+        for (int t = 0; t < ctx->n_fwd; t++) {
+            const int nphi = ctx->cones.n_phi;
+            float dxs[kMaxPhi], dys[kMaxPhi], dzs[kMaxPhi], ixs[kMaxPhi], iys[kMaxPhi], izs[kMaxPhi];
+            float px[kMaxPhi], py[kMaxPhi], pz[kMaxPhi];
+            for (int f = 0; f < nphi; f++) {
+                const float vx = ctx->cones.ux[t][f] / B.beamlet_size[0];
+                const float vy = ctx->cones.uy[t][f] / B.beamlet_size[1];
+                const float vz = ctx->cones.uz[t][f] / B.long_spacing;
+                const float rn = 1.0f / sqrtf(vx * vx + vy * vy + vz * vz);
+                dxs[f] = vx * rn; dys[f] = vy * rn; dzs[f] = vz * rn;
+                ixs[f] = 1.0f / dxs[f]; iys[f] = 1.0f / dys[f]; izs[f] = 1.0f / dzs[f];
+                px[f] = 0.5f; py[f] = 0.5f; pz[f] = 0.0f;
+            }
+            auto step = [&](float c, float d, float inv) {
+                return d > 0.0f ? (ceilf(c + kLargerEps) - c) * inv
+                                : (d < 0.0f ? (floorf(c - kLargerEps) - c) * inv : INFINITY);
+            };
+            int s = 0;
+            while (s < (int)B.long_dim) {
+                for (int f = 0; f < nphi; f++) {
+                    for (int it = 0; it < kMaxStepsPerSlice; it++) {
+                        const float sx = step(px[f], dxs[f], ixs[f]);
+                        const float sy = step(py[f], dys[f], iys[f]);
+                        const float sz = step(pz[f], dzs[f], izs[f]);
+                        const float st = fminf(fminf(sx, sy), sz);
+                        float nx = fmaf(st, dxs[f], px[f]), ny = fmaf(st, dys[f], py[f]), nz = fmaf(st, dzs[f], pz[f]);
+                        const bool flag = fabsf(nz - roundf(nz)) < kEps;
+                        if (flag) {
+                            nx = fmaf(sz, dxs[f], px[f]);
+                            ny = fmaf(sz, dys[f], py[f]);
+                            nz = roundf(fmaf(sz, dzs[f], pz[f]));
+                        }
+                        px[f] = nx; py[f] = ny; pz[f] = nz;
+                        total += kNpix;
+                        if (flag) break;
+                    }
+                }
+                s = std::max((int)roundf(pz[0]), s + 1);
+            }
+        }
+        nseg_out[i] = total;
+    }
+    return FD_OK;
+}
+
+int fd_dosemat_create(uint64_t n_cols, uint64_t est_nnz, fd_dosemat** out) {
+    if (!out || n_cols == 0) {
+        set_error("invalid argument");
+        return FD_ERR_ARG;
+    }
+    fd_dosemat* m = new fd_dosemat();
+    m->n_cols = n_cols;
+    if (est_nnz) {
+        int r = grow_keep(m->columns, 0, est_nnz, nullptr);
+        if (r == FD_OK) r = grow_keep(m->values, 0, est_nnz, nullptr);
+        if (r != FD_OK) {
+            fd_dosemat_destroy(m);
+            return r;
+        }
+    }
+    *out = m;
+    return FD_OK;
+}
+
+int fd_dosemat_destroy(fd_dosemat* m) {
+    if (!m) return FD_OK;
+    m->offsets.release();
+    m->columns.release();
+    m->values.release();
+    delete m;
+    return FD_OK;
+}
+
+int fd_dosemat_add_beam(fd_ctx* ctx, fd_dosemat* m, const fd_beam* beamlets, int n, const float* fluence,
+                        float extent, void* stream) {
+    FD_TRY(check_ctx(ctx));
+    FD_TRY(check_beams(beamlets, n));
+    if (!m || (n > 0 && !fluence)) {
+        set_error("null argument");
+        return FD_ERR_ARG;
+    }
+    const uint64_t nvox = (uint64_t)ctx->vol.dim[0] * ctx->vol.dim[1] * ctx->vol.dim[2];
+    if (nvox != m->n_cols) {
+        set_error("dose matrix column count differs from the context's voxel count");
+        return FD_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const int maxL = max_long_dim(beamlets, n);
+    const uint64_t stride = (uint64_t)maxL * kNpix;
+    uint64_t *off_dst, *col_dst;
+    float* val_dst;
+    if (n == 0) {
+        FD_TRY(append_matrix(m, 0, 0, s, &off_dst, &col_dst, &val_dst));
+        FD_HIP(hipMemsetAsync(off_dst, 0, sizeof(uint64_t), s));
+        return FD_OK;
+    }
+    FD_TRY(ctx->fluence.reserve((size_t)n * kNpix));
+    FD_TRY(ctx->terma.reserve((size_t)n * stride));
+    FD_TRY(ctx->dens_bev.reserve((size_t)n * stride));
+    FD_TRY(ctx->dose_bev.reserve((size_t)n * stride));
+    FD_TRY(upload_beams(ctx, beamlets, n, stride, s));
+    FD_HIP(hipMemcpyAsync(ctx->fluence.ptr, fluence, (size_t)n * kNpix * sizeof(float), hipMemcpyHostToDevice, s));
+    FD_TRY(run_terma(ctx, n, ctx->fluence.ptr, ctx->terma.ptr, ctx->dens_bev.ptr, s));
+    FD_TRY(run_cccs(ctx, n, maxL, ctx->terma.ptr, ctx->dens_bev.ptr, ctx->dose_bev.ptr, stride, s));
+    uint64_t nnz = 0;
+    FD_TRY(run_resample_values(ctx, n, ctx->dose_bev.ptr, extent, &nnz, s));
+    FD_TRY(append_matrix(m, (uint64_t)n, nnz, s, &off_dst, &col_dst, &val_dst));
+    return run_resample_compact(ctx, n, off_dst, col_dst, val_dst, s);
+}
+
+int fd_dosemat_add_csr(fd_dosemat* m, const fd_csr* csr, void* stream) {
+    if (!m || !csr || !csr->offsets) {
+        set_error("null argument");
+        return FD_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    uint64_t *off_dst, *col_dst;
+    float* val_dst;
+    FD_TRY(append_matrix(m, csr->rows, csr->nnz, s, &off_dst, &col_dst, &val_dst));
+    FD_HIP(hipMemcpyAsync(off_dst, csr->offsets, (csr->rows + 1) * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    if (csr->nnz) {
+        FD_HIP(hipMemcpyAsync(col_dst, csr->columns, csr->nnz * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+        FD_HIP(hipMemcpyAsync(val_dst, csr->values, csr->nnz * sizeof(float), hipMemcpyDeviceToDevice, s));
+    }
+    FD_HIP(hipStreamSynchronize(s));
+    return FD_OK;
+}
+
+int fd_dosemat_info(const fd_dosemat* m, uint64_t* n_matrices, uint64_t* total_rows, uint64_t* total_nnz) {
+    if (!m) {
+        set_error("null argument");
+        return FD_ERR_ARG;
+    }
+    uint64_t r = 0;
+    for (uint64_t x : m->rows) r += x;
+    if (n_matrices) *n_matrices = m->rows.size();
+    if (total_rows) *total_rows = r;
+    if (total_nnz) *total_nnz = m->used_nnz;
+    return FD_OK;
+}
+
+int fd_dosemat_matrix_size(const fd_dosemat* m, uint64_t k, uint64_t* rows, uint64_t* nnz) {
+    if (!m || k >= m->rows.size()) {
+        set_error("matrix index out of range");
+        return FD_ERR_ARG;
+    }
+    if (rows) *rows = m->rows[k];
+    if (nnz) *nnz = m->nnz[k];
+    return FD_OK;
+}
+
+int fd_dosemat_copy_matrix(const fd_dosemat* m, uint64_t k, uint64_t* offsets, uint64_t* columns, float* values) {
+    if (!m || k >= m->rows.size()) {
+        set_error("matrix index out of range");
+        return FD_ERR_ARG;
+    }
+    FD_HIP(hipDeviceSynchronize());
+    if (offsets)
+        FD_HIP(hipMemcpy(offsets, m->offsets.ptr + m->off_off[k], (m->rows[k] + 1) * sizeof(uint64_t),
+                         hipMemcpyDeviceToHost));
+    if (m->nnz[k]) {
+        if (columns)
+            FD_HIP(hipMemcpy(columns, m->columns.ptr + m->nnz_off[k], m->nnz[k] * sizeof(uint64_t),
+                             hipMemcpyDeviceToHost));
+        if (values)
+            FD_HIP(hipMemcpy(values, m->values.ptr + m->nnz_off[k], m->nnz[k] * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return FD_OK;
+}
+
+int fd_dosemat_tofile(const fd_dosemat* m, const char* folder) {
+    if (!m || !folder) {
+        set_error("null argument");
+        return FD_ERR_ARG;
+    }
+    const std::string dir(folder);
+    mkdir(dir.c_str(), 0777);
+    const size_t nm = m->rows.size();
+    std::vector<uint64_t> cumu_nnz(nm), cumu_rows(nm);
+    uint64_t a = 0, r = 0;
+    for (size_t i = 0; i < nm; i++) {
+        a += m->nnz[i];
+        r += m->rows[i];
+        cumu_nnz[i] = a;
+        cumu_rows[i] = r;
+    }
+    const std::pair<const std::vector<uint64_t>*, const char*> small[] = {
+        {&m->nnz, "NonZeroElements"},   {&cumu_nnz, "CumuNonZeroElements"}, {&m->rows, "numRowsPerMat"},
+        {&cumu_rows, "CumuNumRowsPerMat"}, {&m->off_off, "OffsetBufferIdx"}};
+    for (const auto& f : small)
+        if (!write_file(dir + "/" + f.second + ".bin", f.first->data(), nm * sizeof(uint64_t))) {
+            set_error("cannot write " + dir + "/" + f.second + ".bin");
+            return FD_ERR_IO;
+        }
+    FD_HIP(hipDeviceSynchronize());
+    std::vector<uint64_t> off(m->used_off), col(m->used_nnz);
+    std::vector<float> val(m->used_nnz);
+    if (m->used_off) FD_HIP(hipMemcpy(off.data(), m->offsets.ptr, m->used_off * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    if (m->used_nnz) {
+        FD_HIP(hipMemcpy(col.data(), m->columns.ptr, m->used_nnz * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        FD_HIP(hipMemcpy(val.data(), m->values.ptr, m->used_nnz * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    if (!write_file(dir + "/offsetsBuffer.bin", off.data(), off.size() * sizeof(uint64_t)) ||
+        !write_file(dir + "/columnsBuffer.bin", col.data(), col.size() * sizeof(uint64_t)) ||
+        !write_file(dir + "/valuesBuffer.bin", val.data(), val.size() * sizeof(float))) {
+        set_error("cannot write the CSR buffers under " + dir);
+        return FD_ERR_IO;
+    }
+    return FD_OK;
+}
+
+int fd_dosemat_fromfile(fd_dosemat* m, const char* folder) {
+    if (!m || !folder) {
+        set_error("null argument");
+        return FD_ERR_ARG;
+    }
+    const std::string dir(folder);
+    std::vector<char> nnz_b, rows_b, oidx_b, off_b, col_b, val_b;
+    if (!read_file(dir + "/NonZeroElements.bin", nnz_b) || !read_file(dir + "/numRowsPerMat.bin", rows_b) ||
+        !read_file(dir + "/OffsetBufferIdx.bin", oidx_b) || !read_file(dir + "/offsetsBuffer.bin", off_b) ||
+        !read_file(dir + "/columnsBuffer.bin", col_b) || !read_file(dir + "/valuesBuffer.bin", val_b)) {
+        set_error("cannot read the dose-matrix folder " + dir);
+        return FD_ERR_IO;
+    }
+    const size_t nm = nnz_b.size() / 8;
+    if (rows_b.size() / 8 != nm || oidx_b.size() / 8 != nm || val_b.size() / 4 != col_b.size() / 8) {
+        set_error("inconsistent dose-matrix files under " + dir);
+        return FD_ERR_IO;
+    }
+    m->rows.assign((const uint64_t*)rows_b.data(), (const uint64_t*)rows_b.data() + nm);
+    m->nnz.assign((const uint64_t*)nnz_b.data(), (const uint64_t*)nnz_b.data() + nm);
+    m->off_off.assign((const uint64_t*)oidx_b.data(), (const uint64_t*)oidx_b.data() + nm);
+    m->nnz_off.resize(nm);
+    uint64_t acc = 0;
+    for (size_t i = 0; i < nm; i++) {
+        m->nnz_off[i] = acc;
+        acc += m->nnz[i];
+    }
+    m->used_off = off_b.size() / 8;
+    m->used_nnz = col_b.size() / 8;
+    if (acc != m->used_nnz) {
+        set_error("NonZeroElements does not add up to the columns buffer");
+        return FD_ERR_IO;
+    }
+    FD_TRY(m->offsets.reserve(std::max<uint64_t>(m->used_off, 1)));
+    FD_TRY(m->columns.reserve(std::max<uint64_t>(m->used_nnz, 1)));
+    FD_TRY(m->values.reserve(std::max<uint64_t>(m->used_nnz, 1)));
+    if (m->used_off) FD_HIP(hipMemcpy(m->offsets.ptr, off_b.data(), off_b.size(), hipMemcpyHostToDevice));
+    if (m->used_nnz) {
+        FD_HIP(hipMemcpy(m->columns.ptr, col_b.data(), col_b.size(), hipMemcpyHostToDevice));
+        FD_HIP(hipMemcpy(m->values.ptr, val_b.data(), val_b.size(), hipMemcpyHostToDevice));
+    }
+    return FD_OK;
+}
+
+int fd_write_fluence_map(const char* path, const uint8_t* flags, uint64_t n_beams, uint64_t per_beam) {
+    if (!path || (!flags && n_beams * per_beam)) {
+        set_error("null argument");
+        return FD_ERR_ARG;
+    }
+    if (!write_file(path, flags, n_beams * per_beam)) {
+        set_error(std::string("cannot open file: ") + path);
+        return FD_ERR_IO;
+    }
+    return FD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,73 @@
+// fd_internal.h -- types shared by the host code and the HIP kernels of libfastdose_hip.
+#pragma once
+
+#include <cstdint>
+#include <cstddef>
+#include <string>
+
+#include "fastdose_hip.h"
+
+namespace fdi {
+
+// Kernel shape constants.  The dose-mode BEV grid is 16 x 16 pixels (subFluenceDim, default
+// 16 in IMRTArgs.cpp:51); a wavefront carries all 256 rays of one cone as 4 rays per lane.
+constexpr int kFmap = 16;
+constexpr int kNpix = kFmap * kFmap;
+constexpr int kWave = 64;
+constexpr int kRaysPerLane = kNpix / kWave;
+constexpr int kMaxPhi = 16;       // one wavefront per azimuth, <= 1024 threads per workgroup
+constexpr int kSuperSampling = 4; // Terma substeps per slice (TermaComp.cu:65)
+constexpr int kResampleSS = 5;    // BEV -> PVCS supersampling per axis (IMRTgeom.cu:352)
+
+// Upper bound on lattice steps of one azimuth within one slice.  A forward cone needs at most
+// 2 + 2 (ls / bs) tan(theta) (|cos phi| + |sin phi|) steps per slice; forward cones must have
+// cos(theta_mid) >= kMinCosTheta (checked when a context is created), which keeps the count far
+// below this cap, so the cap only guarantees that every wavefront's loop ends.
+constexpr int kMaxStepsPerSlice = 1 << 16;
+constexpr double kMinCosTheta = 1e-2;
+
+constexpr float kEps = 1e-4f;        // eps_fastdose (macros.h.in:26)
+constexpr float kLargerEps = 1e-3f;  // larger_eps_fastdose (macros.h.in:27)
+
+// Per-beamlet descriptor the kernels read (the role of d_BEAM_d, beam.cuh:84-133), with the
+// host-rounded FP32 transforms every integer decision is made from.
+struct BeamletDev {
+    float Rf[9];       // PVCS -> BEV rotation, row-major
+    float Ri[9];       // BEV -> PVCS rotation, row-major
+    float src[3];      // source position [cm]
+    float bs[2];       // beamlet_size (BEV pixel pitch at SAD) [cm]
+    float sad;
+    float lim_min;
+    float ls;          // long_spacing [cm]
+    float kx, kz;      // sad / bs.x, sad / bs.y
+    float inv_ls;      // 1 / long_spacing
+    int32_t L;         // long_dim
+    int32_t pad;
+    uint64_t bev_off;  // float offset of this beamlet's BEV volume
+};
+static_assert(sizeof(BeamletDev) % 8 == 0, "BeamletDev alignment");
+
+// Spectrum folded for the Terma loop: w_e = fluence_e * E_e * mu_e and mu_e.
+struct SpectrumDev {
+    int32_t n;
+    float w[FD_MAX_TABLE];
+    float mu[FD_MAX_TABLE];
+};
+
+// Forward cones of the kernel table, heavy (widest) cone first.
+struct ConeTable {
+    int32_t n_fwd;        // forward cones (theta_mid <= pi/2)
+    int32_t n_phi;
+    int32_t theta_idx[FD_MAX_TABLE];  // original cone index of forward slot t
+    float A_a[FD_MAX_TABLE];          // A/a
+    float B_b[FD_MAX_TABLE];          // B/b
+    float a[FD_MAX_TABLE];
+    float b[FD_MAX_TABLE];
+    float ux[FD_MAX_TABLE][kMaxPhi];  // unit direction of (theta, phi), FP64-rounded
+    float uy[FD_MAX_TABLE][kMaxPhi];
+    float uz[FD_MAX_TABLE][kMaxPhi];
+};
+
+void set_error(const std::string& msg);
+
+}  // namespace fdi

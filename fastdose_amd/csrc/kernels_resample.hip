@@ -1,0 +1,322 @@
+// kernels_resample.hip -- BEV -> PVCS resampling of beamlet doses straight into CSR rows.
+//
+// Semantics (IMRTOpt/src/IMRTgeom.cu:370-516): voxel v of the CT grid receives, for beamlet b,
+// the mean over 5^3 supersamples q of the beamlet's BEV dose point-sampled at
+//     u = x_bev kx / y_bev + 8,  w = z_bev kz / y_bev + 8,  d = (y_bev - lim_min) / ls,
+// (x, y, z)_bev = Rf (q - source); samples with u, w outside [0, 16) or d outside [0, L) add 0
+// and still count in the mean.  Only voxels that pass the reference's two culling tests are
+// evaluated: the 8^3 super-voxel containing v has its centre within extent + half super-voxel
+// diagonal of the beamlet axis, and v's centre lies within extent + half voxel diagonal.
+// Exact zeros are dropped (cuSPARSE DenseToSparse semantics, IMRTSparseMatEns.cu:66-165), so
+// the dense nBeamlets x nVoxels intermediate of the reference never exists.
+//
+// Work decomposition.  One workgroup per (z-plane, beamlet).  For each row y of the plane the
+// workgroup bounds the x-range of voxels whose box can reach the beamlet's frustum (six linear
+// inequalities solved in FP64 with a margin); the concatenation of those ranges, in (y, x)
+// order, is the candidate list.  Stage 1 counts candidates, stage 2 evaluates them and compacts
+// nonzeros in voxel order, stage 3 copies them into the CSR arrays.  All decisions (culling
+// tests, which BEV voxel a supersample hits) use FP32 with the operation order of the CPU oracle.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace fdi {
+
+namespace {
+
+constexpr int kRowThreads = 256;
+
+struct PlaneGeom {
+    double b0[3];  // BEV coordinates of the centre of voxel (0, y, z) minus source
+    double g[3];   // BEV step per voxel along x
+};
+
+__device__ __forceinline__ void plane_row(const BeamletDev& B, int y, int z, float vsx, float vsy, float vsz,
+                                          PlaneGeom& pg) {
+    const double px = 0.5 * vsx - (double)B.src[0];
+    const double py = (y + 0.5) * vsy - (double)B.src[1];
+    const double pz = (z + 0.5) * vsz - (double)B.src[2];
+    for (int r = 0; r < 3; r++) {
+        pg.b0[r] = (double)B.Rf[3 * r] * px + (double)B.Rf[3 * r + 1] * py + (double)B.Rf[3 * r + 2] * pz;
+        pg.g[r] = (double)B.Rf[3 * r] * vsx;
+    }
+}
+
+// Candidate x-range [x0, x1) of row (y, z).  rho bounds the distance of any point of a voxel box
+// from its centre plus a rounding margin.
+__device__ __forceinline__ void row_interval(const BeamletDev& B, int y, int z, int X, float vsx, float vsy,
+                                             float vsz, double rho, int& x0, int& x1) {
+    PlaneGeom pg;
+    plane_row(B, y, z, vsx, vsy, vsz, pg);
+    const double cx = 0.5 * kFmap * (double)B.bs[0] / (double)B.sad;
+    const double cz = 0.5 * kFmap * (double)B.bs[1] / (double)B.sad;
+    const double rx = rho * sqrt(1.0 + cx * cx), rz = rho * sqrt(1.0 + cz * cz);
+    const double ylo = (double)B.lim_min - rho;
+    const double yhi = (double)B.lim_min + (double)B.L * (double)B.ls + rho;
+    // alpha + beta * x <= gamma
+    const double al[6] = {-pg.b0[1], pg.b0[1], pg.b0[0] - cx * pg.b0[1], -pg.b0[0] - cx * pg.b0[1],
+                          pg.b0[2] - cz * pg.b0[1], -pg.b0[2] - cz * pg.b0[1]};
+    const double be[6] = {-pg.g[1], pg.g[1], pg.g[0] - cx * pg.g[1], -pg.g[0] - cx * pg.g[1],
+                          pg.g[2] - cz * pg.g[1], -pg.g[2] - cz * pg.g[1]};
+    const double ga[6] = {-ylo, yhi, rx, rx, rz, rz};
+    double lo = -1.0, hi = (double)X;
+    bool empty = false;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const double rhs = ga[k] - al[k];
+        if (be[k] > 0.0) {
+            hi = fmin(hi, rhs / be[k]);
+        } else if (be[k] < 0.0) {
+            lo = fmax(lo, rhs / be[k]);
+        } else if (rhs < 0.0) {
+            empty = true;
+        }
+    }
+    if (empty || !(lo <= hi)) {
+        x0 = 0;
+        x1 = 0;
+        return;
+    }
+    int a = (int)floor(lo) - 1, c = (int)floor(hi) + 2;
+    a = a < 0 ? 0 : a;
+    c = c > X ? X : c;
+    x0 = a;
+    x1 = c > a ? c : a;
+}
+
+__device__ __forceinline__ double rho_of(float vsx, float vsy, float vsz) {
+    return 0.5 * sqrt((double)vsx * vsx + (double)vsy * vsy + (double)vsz * vsz) + 1e-2;
+}
+
+// block-wide exclusive scan of one u32 per thread (blockDim.x <= 1024, multiple of 64)
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wave[wave] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int w = 0; w < nw; w++) {
+            uint32_t t = s_wave[w];
+            s_wave[w] = acc;
+            acc += t;
+        }
+        s_wave[nw] = acc;
+    }
+    __syncthreads();
+    const uint32_t res = s_wave[wave] + x - v;
+    total = s_wave[nw];
+    __syncthreads();
+    return res;
+}
+
+__global__ __launch_bounds__(kRowThreads) void resample_count_kernel(const BeamletDev* __restrict__ beams, int X,
+                                                                     int Y, int Z, float vsx, float vsy, float vsz,
+                                                                     uint32_t* __restrict__ cand_count) {
+    __shared__ uint32_t s_wave[kRowThreads / 64 + 1];
+    const int z = blockIdx.x, b = blockIdx.y;
+    const BeamletDev& B = beams[b];
+    const double rho = rho_of(vsx, vsy, vsz);
+    uint32_t cnt = 0;
+    for (int y = threadIdx.x; y < Y; y += blockDim.x) {
+        int x0, x1;
+        row_interval(B, y, z, X, vsx, vsy, vsz, rho, x0, x1);
+        cnt += (uint32_t)(x1 - x0);
+    }
+    uint32_t total;
+    block_exclusive_scan(cnt, s_wave, total);
+    if (threadIdx.x == 0) cand_count[(size_t)b * Z + z] = total;
+}
+
+// value of voxel (x, y, z) for beamlet B; decisions in FP32 with the oracle's order
+__device__ float voxel_value(const BeamletDev& B, const float* __restrict__ dose, int x, int y, int z, float vsx,
+                             float vsy, float vsz, float extent, float half_diag, float half_sdiag) {
+    const float* M = B.Rf;
+    const float s0 = B.src[0], s1 = B.src[1], s2 = B.src[2];
+    // super-voxel test (d_superVoxelInterp, IMRTgeom.cu:471-516)
+    {
+        const float cx = (8 * vsx) * ((x >> 3) + 0.5f) - s0;
+        const float cy = (8 * vsy) * ((y >> 3) + 0.5f) - s1;
+        const float cz = (8 * vsz) * ((z >> 3) + 0.5f) - s2;
+        const float bx = fmaf(M[2], cz, fmaf(M[1], cy, M[0] * cx));
+        const float bz = fmaf(M[8], cz, fmaf(M[7], cy, M[6] * cx));
+        const float dist = sqrtf(bx * bx + bz * bz);
+        if (!(dist < extent + half_sdiag)) return 0.0f;
+    }
+    // voxel-centre test (IMRTgeom.cu:410-421)
+    {
+        const float cx = (x + 0.5f) * vsx - s0;
+        const float cy = (y + 0.5f) * vsy - s1;
+        const float cz = (z + 0.5f) * vsz - s2;
+        const float bx = fmaf(M[2], cz, fmaf(M[1], cy, M[0] * cx));
+        const float bz = fmaf(M[8], cz, fmaf(M[7], cy, M[6] * cx));
+        const float dist = sqrtf(bx * bx + bz * bz);
+        if (dist > half_diag + extent) return 0.0f;
+    }
+    const float inv_ss = 1.0f / kResampleSS;
+    const float Lf = (float)B.L;
+    const float half = 0.5f * kFmap;
+    const float* __restrict__ D = dose + B.bev_off;
+    // per-axis sample coordinates minus source
+    float qx[kResampleSS], qy[kResampleSS], qz[kResampleSS];
+#pragma unroll
+    for (int i = 0; i < kResampleSS; i++) {
+        const float o = (i + 0.5f) * inv_ss;
+        qx[i] = ((float)x + o) * vsx - s0;
+        qy[i] = ((float)y + o) * vsy - s1;
+        qz[i] = ((float)z + o) * vsz - s2;
+    }
+    float acc = 0.0f;
+    for (int kk = 0; kk < kResampleSS; kk++) {
+        for (int jj = 0; jj < kResampleSS; jj++) {
+#pragma unroll
+            for (int ii = 0; ii < kResampleSS; ii++) {
+                const float bx = fmaf(M[2], qz[kk], fmaf(M[1], qy[jj], M[0] * qx[ii]));
+                const float by = fmaf(M[5], qz[kk], fmaf(M[4], qy[jj], M[3] * qx[ii]));
+                const float bz = fmaf(M[8], qz[kk], fmaf(M[7], qy[jj], M[6] * qx[ii]));
+                const float dd = (by - B.lim_min) * B.inv_ls;
+                if (!(dd >= 0.0f && dd < Lf)) continue;
+                const float rinv = 1.0f / by;
+                const float u = fmaf(bx * B.kx, rinv, half);
+                const float w = fmaf(bz * B.kz, rinv, half);
+                if (!(u >= 0.0f && u < (float)kFmap && w >= 0.0f && w < (float)kFmap)) continue;
+                acc += D[(size_t)(int)dd * kNpix + (int)w * kFmap + (int)u];
+            }
+        }
+    }
+    return acc * (1.0f / (kResampleSS * kResampleSS * kResampleSS));
+}
+
+__global__ __launch_bounds__(kRowThreads) void resample_values_kernel(
+    const BeamletDev* __restrict__ beams, int X, int Y, int Z, float vsx, float vsy, float vsz, float extent,
+    const float* __restrict__ dose, const uint64_t* __restrict__ cand_off, uint64_t* __restrict__ tmp_cols,
+    float* __restrict__ tmp_vals, uint32_t* __restrict__ nnz_count) {
+    extern __shared__ uint32_t s_rows[];  // x0[Y], off[Y + 1]
+    __shared__ uint32_t s_wave[kRowThreads / 64 + 1];
+    uint32_t* s_x0 = s_rows;
+    uint32_t* s_off = s_rows + Y;
+    const int z = blockIdx.x, b = blockIdx.y;
+    const BeamletDev& B = beams[b];
+    const double rho = rho_of(vsx, vsy, vsz);
+
+    // row ranges and their prefix (rows handled in chunks of blockDim)
+    uint32_t carry = 0;
+    for (int y0 = 0; y0 < Y; y0 += blockDim.x) {
+        const int y = y0 + threadIdx.x;
+        uint32_t len = 0;
+        if (y < Y) {
+            int x0, x1;
+            row_interval(B, y, z, X, vsx, vsy, vsz, rho, x0, x1);
+            s_x0[y] = (uint32_t)x0;
+            len = (uint32_t)(x1 - x0);
+        }
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan(len, s_wave, tot);
+        if (y < Y) s_off[y] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) s_off[Y] = carry;
+    __syncthreads();
+    const uint32_t total = carry;
+    const uint64_t base = cand_off[(size_t)b * Z + z];
+
+    const float half_diag = sqrtf(vsx * vsx + vsy * vsy + vsz * vsz) * 0.5f;
+    const float svx = 8 * vsx, svy = 8 * vsy, svz = 8 * vsz;
+    const float half_sdiag = sqrtf(svx * svx + svy * svy + svz * svz) * 0.5f;
+
+    uint32_t written = 0;
+    for (uint32_t c0 = 0; c0 < total; c0 += blockDim.x) {
+        const uint32_t c = c0 + threadIdx.x;
+        float v = 0.0f;
+        int x = 0, y = 0;
+        if (c < total) {
+            // row containing candidate c: largest y with s_off[y] <= c
+            int lo = 0, hi = Y - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_off[mid] <= c) lo = mid; else hi = mid - 1;
+            }
+            y = lo;
+            x = (int)(s_x0[y] + (c - s_off[y]));
+            v = voxel_value(B, dose, x, y, z, vsx, vsy, vsz, extent, half_diag, half_sdiag);
+        }
+        const uint32_t nz = (c < total && v != 0.0f) ? 1u : 0u;
+        uint32_t tot;
+        const uint32_t pos = block_exclusive_scan(nz, s_wave, tot);
+        if (nz) {
+            const uint64_t o = base + written + pos;
+            tmp_cols[o] = (uint64_t)x + (uint64_t)X * ((uint64_t)y + (uint64_t)Y * (uint64_t)z);
+            tmp_vals[o] = v;
+        }
+        written += tot;
+    }
+    if (threadIdx.x == 0) nnz_count[(size_t)b * Z + z] = written;
+}
+
+__global__ __launch_bounds__(256) void resample_compact_kernel(int Z, const uint64_t* __restrict__ cand_off,
+                                                               const uint64_t* __restrict__ nnz_off,
+                                                               const uint32_t* __restrict__ nnz_count,
+                                                               const uint64_t* __restrict__ tmp_cols,
+                                                               const float* __restrict__ tmp_vals,
+                                                               uint64_t* __restrict__ columns,
+                                                               float* __restrict__ values) {
+    const size_t seg = (size_t)blockIdx.y * Z + blockIdx.x;
+    const uint32_t cnt = nnz_count[seg];
+    const uint64_t src = cand_off[seg], dst = nnz_off[seg];
+    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+        columns[dst + i] = tmp_cols[src + i];
+        values[dst + i] = tmp_vals[src + i];
+    }
+}
+
+__global__ void row_offsets_kernel(int n, int Z, const uint64_t* __restrict__ nnz_off,
+                                   const uint64_t* __restrict__ total, uint64_t* __restrict__ offsets) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < n) offsets[b] = nnz_off[(size_t)b * Z];
+    if (b == n) offsets[n] = total[0];
+}
+
+}  // namespace
+
+hipError_t launch_resample_count(const BeamletDev* beams, int n, const fd_volume& vol, uint32_t* cand_count,
+                                 hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    dim3 grid(vol.dim[2], n), block(kRowThreads);
+    hipLaunchKernelGGL(resample_count_kernel, grid, block, 0, stream, beams, (int)vol.dim[0], (int)vol.dim[1],
+                       (int)vol.dim[2], vol.voxel_size[0], vol.voxel_size[1], vol.voxel_size[2], cand_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_resample_values(const BeamletDev* beams, int n, const fd_volume& vol, float extent,
+                                  const float* dose_bev, const uint64_t* cand_off, uint64_t* tmp_cols,
+                                  float* tmp_vals, uint32_t* nnz_count, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    dim3 grid(vol.dim[2], n), block(kRowThreads);
+    const size_t shm = (2 * (size_t)vol.dim[1] + 1) * sizeof(uint32_t);
+    hipLaunchKernelGGL(resample_values_kernel, grid, block, shm, stream, beams, (int)vol.dim[0], (int)vol.dim[1],
+                       (int)vol.dim[2], vol.voxel_size[0], vol.voxel_size[1], vol.voxel_size[2], extent, dose_bev,
+                       cand_off, tmp_cols, tmp_vals, nnz_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_resample_compact(int n, int Z, const uint64_t* cand_off, const uint64_t* nnz_off,
+                                   const uint32_t* nnz_count, const uint64_t* tmp_cols, const float* tmp_vals,
+                                   uint64_t* offsets, uint64_t* columns, float* values, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    dim3 grid(Z, n), block(256);
+    hipLaunchKernelGGL(resample_compact_kernel, grid, block, 0, stream, Z, cand_off, nnz_off, nnz_count, tmp_cols,
+                       tmp_vals, columns, values);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // nnz_off has n*Z entries followed by the total at index n*Z
+    hipLaunchKernelGGL(row_offsets_kernel, dim3((n + 1 + 255) / 256), dim3(256), 0, stream, n, Z, nnz_off,
+                       nnz_off + (size_t)n * Z, offsets);
+    return hipGetLastError();
+}
+
+}  // namespace fdi

@@ -1,0 +1,238 @@
+/* fastdose_hip.h -- C-ABI of the MI355X-native CCCS beamlet-dose library (libfastdose_hip.so).
+ *
+ * This boundary replaces the C++/CUDA entry points that FastDose's dose mode (`IMRT --mode 0`)
+ * links against.  Each function below names the reference interface it stands in for
+ * (paths relative to the reference repository).  Everything here is plain C: POD structs,
+ * raw pointers and sizes, `void*` for a hipStream_t.  No torch or HIP types appear.
+ *
+ * Conventions.
+ *   - Every function returns an int status: FD_OK (0) on success, a negative FD_ERR_* code on
+ *     failure.  The library never calls exit().  fd_last_error() returns a message for the
+ *     last failure on the calling thread.  (The reference returns bool with true = failure and
+ *     exits on CUDA errors.)
+ *   - "dev" pointers are device (HBM) pointers; everything else is host memory.
+ *   - Beam/beamlet arrays (fd_beam) are host arrays; the library uploads what its kernels need.
+ *   - A context (fd_ctx) owns the device-resident density and the kernel/spectrum tables, so
+ *     several contexts may coexist in one process (the reference keeps these tables in
+ *     process-global __constant__ memory).  Calls on one context must not overlap in time.
+ *   - BEV volumes of n beamlets live in one buffer: beamlet i occupies floats
+ *     [i*stride, i*stride + long_dim_i*npix), slice-major, pixel index ix + fmap_x*iy.
+ */
+#ifndef FASTDOSE_HIP_H
+#define FASTDOSE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FD_MAX_TABLE 24 /* MAX_KERNEL_NUM / MAX_THETA_ANGLES (spectrum.cuh:7, kernel.cuh:7) */
+
+enum {
+    FD_OK = 0,
+    FD_ERR_ARG = -1,         /* invalid argument / size */
+    FD_ERR_HIP = -2,         /* HIP runtime error */
+    FD_ERR_GEOM = -3,        /* beam misses the bbox, canonical-form search failed */
+    FD_ERR_IO = -4,          /* file could not be read or written */
+    FD_ERR_CAPACITY = -5,    /* caller buffer too small; the needed size is reported */
+    FD_ERR_UNSUPPORTED = -6  /* shape the kernels do not implement (e.g. fmap != 16x16) */
+};
+
+/* Option bits for fd_ctx_set_options. */
+enum {
+    FD_OPT_EXACT_G = 1,      /* use g = (1-e^-x)/x (KernelGen/derivation.md:248) instead of the
+                                reference's g == 1 (DoseComp.cu:167-175) */
+    FD_OPT_RPL_FP64 = 2,     /* accumulate the Terma radiological path in FP64 on rays longer
+                                than 256 voxels */
+    FD_OPT_TIMING = 4        /* record HIP events around every kernel launch (fd_ctx_timings) */
+};
+
+/* Kernel slots reported by fd_ctx_timings. */
+enum {
+    FD_T_TERMA = 0,
+    FD_T_CCCS = 1,
+    FD_T_CONE_SUM = 2,
+    FD_T_RESAMPLE_COUNT = 3,
+    FD_T_RESAMPLE_VALUES = 4,
+    FD_T_RESAMPLE_COMPACT = 5,
+    FD_T_COUNT = 6
+};
+
+/* DENSITY_h geometry (density.cuh:9-28): voxel size [cm], volume dims (x fastest), bbox. */
+typedef struct fd_volume {
+    float voxel_size[3];
+    uint32_t dim[3];
+    uint32_t bbox_start[3];
+    uint32_t bbox_dim[3];
+} fd_volume;
+
+/* BEAM_h (beam.cuh:10-34) without the fluence vector: angles = (gantry/theta, couch/phi,
+ * collimator) in rad; lim_min/lim_max/long_dim/source are filled by fd_calc_range. */
+typedef struct fd_beam {
+    float isocenter[3];
+    float beamlet_size[2];
+    uint32_t fmap_size[2];
+    float sad;
+    float angles[3];
+    float long_spacing;
+    float lim_min;
+    float lim_max;
+    uint32_t long_dim;
+    float source[3];
+} fd_beam;
+
+/* SPECTRUM_h (spectrum.cuh:10-20): fluence normalised to sum 1. */
+typedef struct fd_spectrum {
+    int32_t n;
+    float energy[FD_MAX_TABLE];
+    float fluence[FD_MAX_TABLE];
+    float mu[FD_MAX_TABLE];
+    float mu_en[FD_MAX_TABLE];
+} fd_spectrum;
+
+/* KERNEL_h (kernel.cuh:10-25): per zenith cone A,a,B,b and theta bounds; phi grid. */
+typedef struct fd_kernel {
+    int32_t n_theta;
+    int32_t n_phi;
+    float theta_begin[FD_MAX_TABLE];
+    float theta_end[FD_MAX_TABLE];
+    float theta_mid[FD_MAX_TABLE];
+    float A[FD_MAX_TABLE];
+    float a[FD_MAX_TABLE];
+    float B[FD_MAX_TABLE];
+    float b[FD_MAX_TABLE];
+    float phi[FD_MAX_TABLE];
+} fd_kernel;
+
+/* Scalar fields of IMRT::BeamBundle (IMRTBeamBundle.cuh:9-30). */
+typedef struct fd_bundle {
+    float isocenter[3];
+    float beamlet_size[2];
+    int32_t fluence_dim[2];
+    float sad;
+    float angles[3];
+    float long_spacing;
+    int32_t sub_dim[2];
+    int32_t sub_on[2];
+} fd_bundle;
+
+/* A CSR matrix (rows = beamlets, columns = PVCS voxels x + X*(y + Y*z)), 0-based, 64-bit
+ * indices, ascending columns, exact zeros dropped -- the layout cuSPARSE DenseToSparse
+ * produces in MatCSREnsemble::addMat (IMRTSparseMatEns.cu:66-165). */
+typedef struct fd_csr {
+    uint64_t rows;
+    uint64_t nnz;
+    uint64_t capacity;   /* entries allocated in columns/values */
+    uint64_t* offsets;   /* dev, rows + 1 */
+    uint64_t* columns;   /* dev, capacity */
+    float* values;       /* dev, capacity */
+} fd_csr;
+
+typedef struct fd_ctx fd_ctx;
+typedef struct fd_dosemat fd_dosemat;
+
+const char* fd_last_error(void);
+const char* fd_version(void);
+
+/* ------------------------------------------------------------------ host geometry (no GPU) */
+
+/* BEAM_h::calc_range (beam.cu:134-221): source, lim_min, lim_max, long_dim. FD_ERR_GEOM if the
+ * beam misses the bbox. */
+int fd_calc_range(fd_beam* beam, const fd_volume* vol);
+
+/* BeamBundle::beamletInit (IMRTBeamBundle.cu:341-569): canonical beamlet (ix, iy) of a bundle,
+ * its sub-fluence (sub_dim[0]*sub_dim[1] floats, centre sub_on block = 1) and calc_range. */
+int fd_beamlet_init(const fd_bundle* bundle, int ix, int iy, const fd_volume* vol, fd_beam* out,
+                    float* fluence_out);
+
+/* BeamBundle::beamletsInit (IMRTBeamBundle.cu:319-338): every flagged beamlet, row order
+ * (iy outer, ix inner).  out/fluence_out hold room for all fluence_dim^2 beamlets. */
+int fd_beamlets_init(const fd_bundle* bundle, const uint8_t* flags, const fd_volume* vol, fd_beam* out,
+                     float* fluence_out, int* n_out);
+
+/* BeamBundleInit's isocenter (IMRTBeamBundle.cu:36-65): PTV centroid [cm]. */
+int fd_ptv_isocenter(const uint8_t* mask, const fd_volume* vol, float iso[3]);
+
+/* beamletFlagInit / d_beamletFlagInit (IMRTBeamBundle.cu:130-316) on the host: flags of the
+ * fluence_dim^2 beamlets that intersect the PTV mask. */
+int fd_beamlet_flags_host(const fd_bundle* bundle, const uint8_t* ptv_mask, const fd_volume* vol,
+                          uint8_t* flags);
+
+/* SPECTRUM_h::read_spectrum_file (spectrum.cu:14-60). */
+int fd_read_spectrum(const char* path, fd_spectrum* out);
+/* KERNEL_h::read_kernel_file (kernel.cu:13-72). */
+int fd_read_kernel(const char* path, int n_phi, fd_kernel* out);
+
+/* ------------------------------------------------------------------ device context */
+
+/* density_h2d + bind_spectrum + bind_kernel (density.cu:23-58, TermaComp.cu:67-78,
+ * DoseComp.cu:25-38): uploads the density volume and tables once. */
+int fd_ctx_create(int device, const fd_volume* vol, const float* density_host, const fd_spectrum* spectrum,
+                  const fd_kernel* kernel, fd_ctx** out);
+int fd_ctx_destroy(fd_ctx* ctx);
+int fd_ctx_set_options(fd_ctx* ctx, int options);
+/* Milliseconds each kernel slot took in the last call that launched it (FD_OPT_TIMING set),
+ * measured with HIP events on the launch stream; -1 for slots not run.  Plays the role of the
+ * reference's cudaEvent timers (TermaComp.cu:310-333, DoseComp.cu:449-472). */
+int fd_ctx_timings(fd_ctx* ctx, float* ms, int n);
+/* Device pointer of the resident density volume (x fastest). */
+int fd_ctx_density(fd_ctx* ctx, const float** density_dev);
+
+/* TermaComputeCollective (TermaComp.cuh:7-17): Terma and mean density along the BEV rays of
+ * n beamlets.  fluence_dev: n * npix floats. */
+int fd_terma(fd_ctx* ctx, const fd_beam* beams, int n, const float* fluence_dev, float* terma_dev,
+             float* density_bev_dev, uint64_t stride, void* stream);
+
+/* DoseComputeCollective (DoseComp.cuh:9-19): CCCS recursion over the forward cones; writes
+ * (does not accumulate) the BEV dose of each beamlet. */
+int fd_cccs(fd_ctx* ctx, const fd_beam* beams, int n, const float* terma_dev, const float* density_bev_dev,
+            float* dose_bev_dev, uint64_t stride, void* stream);
+
+/* BEV2PVCSInterp + MatCSREnsemble::addMat (IMRTgeom.cuh:14-33, IMRTDoseMatEns.cuh:30): 5^3
+ * point-sampled BEV -> PVCS resampling of n beamlets straight into CSR rows (no dense
+ * intermediate).  out->offsets must hold n+1 entries; if out->capacity < nnz the call returns
+ * FD_ERR_CAPACITY with out->nnz set (two-phase use). */
+int fd_resample_csr(fd_ctx* ctx, const fd_beam* beams, int n, const float* dose_bev_dev, uint64_t stride,
+                    float extent, fd_csr* out, void* stream);
+
+/* cusparseDenseToSparse as used by MatCSREnsemble::addMat (IMRTSparseMatEns.cu:78-161):
+ * row-major dense (dev) -> CSR.  Same two-phase capacity rule as fd_resample_csr. */
+int fd_dense_to_csr(const float* dense_dev, uint64_t rows, uint64_t cols, fd_csr* out, void* stream);
+
+/* Number of (ray, segment) steps of the CCCS march of each beamlet (host count of the same FP32
+ * lattice the kernel walks); used for the FLOP accounting and load balancing. */
+int fd_cccs_segments(fd_ctx* ctx, const fd_beam* beams, int n, int64_t* nseg_out);
+
+/* ------------------------------------------------------------------ dose-loading matrix */
+
+/* MatCSREnsemble (IMRTDoseMatEns.cuh:12-60): one CSR matrix per beam, n_cols = PVCS voxels. */
+int fd_dosemat_create(uint64_t n_cols, uint64_t est_nnz, fd_dosemat** out);
+int fd_dosemat_destroy(fd_dosemat* m);
+/* One iteration of DoseMatConstruction (IMRTDoseMat.cu:183-268): Terma -> CCCS -> resample of
+ * one beam's n beamlets, appended as the next matrix.  fluence: n*npix host floats, as
+ * fd_beamlet_init writes them. */
+int fd_dosemat_add_beam(fd_ctx* ctx, fd_dosemat* m, const fd_beam* beamlets, int n, const float* fluence,
+                        float extent, void* stream);
+/* Appends an existing device CSR (copied). */
+int fd_dosemat_add_csr(fd_dosemat* m, const fd_csr* csr, void* stream);
+int fd_dosemat_info(const fd_dosemat* m, uint64_t* n_matrices, uint64_t* total_rows, uint64_t* total_nnz);
+/* Host copy of matrix k: offsets (rows+1), columns, values; sizes from fd_dosemat_matrix_size. */
+int fd_dosemat_matrix_size(const fd_dosemat* m, uint64_t k, uint64_t* rows, uint64_t* nnz);
+int fd_dosemat_copy_matrix(const fd_dosemat* m, uint64_t k, uint64_t* offsets, uint64_t* columns,
+                           float* values);
+/* MatCSREnsemble::tofile (IMRTSparseMatEns.cu:168-239): the doseMatFolder layout read by
+ * scripts/readDose.m. */
+int fd_dosemat_tofile(const fd_dosemat* m, const char* folder);
+/* MatCSREnsemble::fromfile (IMRTSparseMatEns.cu:242-336). */
+int fd_dosemat_fromfile(fd_dosemat* m, const char* folder);
+
+/* beamletFlagSave (IMRTBeamBundle.cu:572-601): u8 flags, n_beams * per_beam. */
+int fd_write_fluence_map(const char* path, const uint8_t* flags, uint64_t n_beams, uint64_t per_beam);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTDOSE_HIP_H */

@@ -1,0 +1,213 @@
+"""HIP path vs the FP64 oracle, through the C-ABI (needs an MI355X).
+
+Bar: BEV geometry and CSR column sets bit-exact; Terma, CCCS dose and resampled dose within 0.5%
+of each beamlet's maximum (|D_gpu - D_fp64| <= 0.005 max_b D_fp64).
+"""
+import numpy as np
+import pytest
+
+import fastdose_amd as fd
+import oracle as orc
+from fastdose_amd import synth
+
+from gpu_util import beamlets_of, dense_diff, gpu_stages, log, oracle_beamlet, row
+
+pytestmark = pytest.mark.gpu
+
+TOL = 0.005  # per-beamlet max-dose tolerance
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+def check_against_oracle(cfg, beams, flu, g, idx, flags=0, stage_isolated=True):
+    worst = {"terma": 0.0, "dose": 0.0, "csr": 0.0}
+    for i in idx:
+        b = beams[i]
+        L = b.long_dim
+        ref = oracle_beamlet(cfg, b, flu[i], flags)
+        T, R, D = g["terma"][i, :L], g["dens"][i, :L], g["dose"][i, :L]
+        et = np.abs(T - ref["terma"]).max() / ref["terma"].max()
+        er = np.abs(R - ref["dens"]).max() / max(ref["dens"].max(), 1e-30)
+        ed = np.abs(D - ref["dose"]).max() / ref["dose"].max()
+        cols, vals = row(g, i)
+        assert np.all(np.diff(cols.astype(np.int64)) > 0), "columns not ascending"
+        ec, sym = dense_diff(cols, vals, ref["cols"], ref["vals"])
+        ec /= max(ref["vals"].max(), 1e-30)
+        worst["terma"] = max(worst["terma"], et)
+        worst["dose"] = max(worst["dose"], ed)
+        worst["csr"] = max(worst["csr"], ec)
+        assert et < 1e-4 and er < 1e-4, (i, et, er)
+        assert ed <= TOL, (i, ed)
+        assert ec <= TOL, (i, ec)
+        # index sets: identical except for entries that are structurally ~0 on one side
+        if len(sym):
+            u = np.union1d(cols, ref["cols"])
+            tiny = 1e-6 * ref["vals"].max()
+            vg = dict(zip(cols.tolist(), vals.tolist()))
+            vr = dict(zip(ref["cols"].tolist(), ref["vals"].tolist()))
+            for c in sym.tolist():
+                assert max(vg.get(c, 0.0), vr.get(c, 0.0)) <= tiny, (i, c)
+            assert len(sym) <= 1e-4 * len(u) + 2
+        if stage_isolated:
+            # the resampler on the GPU's own BEV dose: decisions and index sets bit-exact
+            c2, v2, _ = orc.resample(orc.cast(orc.Beam, b), D.astype(np.float64), orc.cast(orc.Volume, cfg.volume),
+                                     cfg.extent)
+            assert np.array_equal(cols.astype(np.int64), c2), i
+            assert np.allclose(vals, v2, rtol=2e-6, atol=1e-30 + 1e-7 * v2.max())
+    log(f"worst rel errors {worst}")
+    return worst
+
+
+def test_small_hetero(torch_cuda):
+    cfg = synth.small_config(seed=7, dim=(24, 28, 20), fluence_dim=2)
+    beams, flu = beamlets_of(cfg)
+    log("small_hetero ctx")
+    with fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel) as ctx:
+        g = gpu_stages(ctx, beams, flu, cfg.extent)
+    check_against_oracle(cfg, beams, flu, g, range(len(beams)))
+
+
+def test_small_exact_g(torch_cuda):
+    cfg = synth.small_config(seed=8, dim=(22, 22, 26), fluence_dim=2, angles=(-50.0, 70.0, 30.0))
+    beams, flu = beamlets_of(cfg)
+    log("small_exact_g ctx")
+    with fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel, options=fd.FD_OPT_EXACT_G) as ctx:
+        g = gpu_stages(ctx, beams, flu, cfg.extent)
+    check_against_oracle(cfg, beams, flu, g, range(len(beams)), flags=orc.EXACT_G)
+
+
+def test_config1_all_beamlets(torch_cuda):
+    cfg = synth.config1()
+    beams, flu = beamlets_of(cfg)
+    assert len(beams) == 64
+    log("config1 ctx")
+    with fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel) as ctx:
+        g = gpu_stages(ctx, beams, flu, cfg.extent)
+    check_against_oracle(cfg, beams, flu, g, range(0, 64, 3))
+
+
+def test_config2_sampled_beamlets(torch_cuda):
+    cfg = synth.config2()
+    beams, flu = beamlets_of(cfg)
+    assert len(beams) == 256
+    log("config2 ctx")
+    with fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel) as ctx:
+        g = gpu_stages(ctx, beams, flu, cfg.extent)
+    # size-independent properties on every row
+    off = g["offsets"]
+    assert off[0] == 0 and np.all(np.diff(off) > 0)
+    check_against_oracle(cfg, beams, flu, g, [0, 37, 120, 135, 255], stage_isolated=True)
+
+
+def test_segment_count_matches_oracle(torch_cuda):
+    cfg = synth.config1()
+    beams, flu = beamlets_of(cfg)
+    with fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel) as ctx:
+        nseg = ctx.cccs_segments(beams)
+    k = orc.cast(orc.Kernel, cfg.kernel)
+    for i in (0, 9, 63):
+        assert nseg[i] == orc.cccs_count(orc.cast(orc.Beam, beams[i]), k)
+
+
+def test_deterministic_and_batch_independent(torch_cuda):
+    """Each beamlet's result is the same alone and inside a batch, bit for bit."""
+    cfg = synth.small_config(seed=9, dim=(26, 24, 22), fluence_dim=3)
+    beams, flu = beamlets_of(cfg)
+    with fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel) as ctx:
+        g1 = gpu_stages(ctx, beams, flu, cfg.extent)
+        g2 = gpu_stages(ctx, beams, flu, cfg.extent)
+        for k in (0, 4, len(beams) - 1):
+            alone = gpu_stages(ctx, fd.beam_array([beams[k]]), flu[k:k + 1], cfg.extent)
+            L = beams[k].long_dim
+            assert np.array_equal(alone["dose"][0, :L], g1["dose"][k, :L])
+            c1, v1 = row(g1, k)
+            c2, v2 = row(alone, 0)
+            assert np.array_equal(c1, c2) and np.array_equal(v1, v2)
+    for key in ("terma", "dose", "cols", "vals", "offsets"):
+        assert np.array_equal(g1[key], g2[key]), key
+
+
+def test_dense_to_csr_known_and_random(torch_cuda):
+    torch = torch_cuda
+    dense = np.array([[0, 0, 1, 0, 2, 0], [2, 0, 0, 1, 0, 0], [0, 0, 1, 2, 0, 1], [1, 0, 1, 0, 0, 0],
+                      [1, 1, 0, 0, 0, 1]], dtype=np.float32)
+    off, col, val = fd.dense_to_csr(torch.from_numpy(dense).cuda())
+    assert off.cpu().tolist() == [0, 2, 4, 7, 9, 12]
+    assert col.cpu().tolist() == [2, 4, 0, 3, 2, 3, 5, 0, 2, 0, 1, 5]
+    assert val.cpu().tolist() == [1, 2, 2, 1, 1, 2, 1, 1, 1, 1, 1, 1]
+    dense4 = np.array([[0, 0, 1, 2], [1, 0, 0, 1], [0, 0, 1, 0], [2, 0, 0, 0]], dtype=np.float32)
+    off, col, val = fd.dense_to_csr(torch.from_numpy(dense4).cuda())
+    assert off.cpu().tolist() == [0, 2, 4, 5, 6]
+    assert col.cpu().tolist() == [2, 3, 0, 3, 2, 0]
+    rng = np.random.default_rng(0)
+    for rows, cols in [(1, 1), (3, 1000), (37, 5003), (0, 10), (5, 0)]:
+        d = rng.uniform(size=(rows, cols)).astype(np.float32)
+        d[d < 0.7] = 0.0
+        off, col, val = fd.dense_to_csr(torch.from_numpy(d).cuda())
+        o2, c2, v2 = orc.dense_to_csr(d) if rows and cols else (np.zeros(rows + 1, np.uint64), [], [])
+        assert np.array_equal(off.cpu().numpy().astype(np.uint64), np.asarray(o2, dtype=np.uint64))
+        assert np.array_equal(col.cpu().numpy(), np.asarray(c2, dtype=np.int64))
+        assert np.array_equal(val.cpu().numpy(), np.asarray(v2, dtype=np.float32))
+
+
+def test_dosemat_files_roundtrip(torch_cuda, tmp_path):
+    """DoseMatConstruction -> tofile -> readDose.m semantics -> fromfile."""
+    cfg = synth.small_config(seed=10, dim=(24, 24, 24), fluence_dim=2)
+    b2 = synth.make_bundle(cfg.bundles[0].isocenter, (120.0, -30.0, 0.0), fluence_dim=2)
+    cfg.bundles.append(b2)
+    cfg.flags.append(np.array([1, 0, 1, 1], dtype=np.uint8))
+    with fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel) as ctx:
+        dm = fd.dose_mat_construction(ctx, cfg.bundles, cfg.flags, cfg.extent)
+        nm, rows, nnz = dm.info()
+        assert nm == 2 and rows == 4 + 3
+        folder = tmp_path / "doseMatFolder"
+        dm.tofile(folder)
+        fd.write_fluence_map(tmp_path / "fluenceMap.bin", cfg.flags)
+        mats = fd.read_dosemat_folder(folder)
+        for k in range(2):
+            off, col, val = dm.matrix(k)
+            assert np.array_equal(mats[k][0], off)
+            assert np.array_equal(mats[k][1], col)
+            assert np.array_equal(mats[k][2], val)
+        # the file set of MatCSREnsemble::tofile
+        for name in ("NonZeroElements", "CumuNonZeroElements", "numRowsPerMat", "CumuNumRowsPerMat",
+                     "OffsetBufferIdx", "offsetsBuffer", "columnsBuffer", "valuesBuffer"):
+            assert (folder / f"{name}.bin").exists()
+        assert np.fromfile(folder / "OffsetBufferIdx.bin", dtype=np.uint64).tolist() == [0, 5]
+        assert np.fromfile(tmp_path / "fluenceMap.bin", dtype=np.uint8).tolist() == [1, 1, 1, 1, 1, 0, 1, 1]
+        dm2 = fd.DoseMat(dm.n_cols)
+        dm2.fromfile(folder)
+        assert dm2.info() == dm.info()
+        for k in range(2):
+            a, b = dm.matrix(k), dm2.matrix(k)
+            assert all(np.array_equal(x, y) for x, y in zip(a, b))
+        # the dose matrix rows equal the split-stage rows
+        beams, flu = fd.beamlets_init(cfg.bundles[0], cfg.flags[0], cfg.volume)
+        g = gpu_stages(ctx, beams, flu, cfg.extent)
+        off, col, val = dm.matrix(0)
+        assert np.array_equal(off.astype(np.int64), g["offsets"])
+        assert np.array_equal(col.astype(np.int64), g["cols"]) and np.array_equal(val, g["vals"])
+
+
+def test_error_paths(torch_cuda):
+    torch = torch_cuda
+    cfg = synth.small_config(seed=11, dim=(16, 16, 16), fluence_dim=1)
+    beams, flu = beamlets_of(cfg)
+    with fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel) as ctx:
+        stride = beams[0].long_dim * 256
+        T = torch.zeros(stride, device="cuda")
+        with pytest.raises(fd.FastDoseError):
+            ctx.terma(beams, torch.from_numpy(flu).cuda(), T, T, stride - 1)  # stride too small
+        bad = fd.Beam()
+        bad.fmap_size[0] = bad.fmap_size[1] = 8
+        bad.long_dim = 4
+        with pytest.raises(fd.FastDoseError) as e:
+            ctx.cccs([bad], T, T, T, stride)
+        assert e.value.code == -6
