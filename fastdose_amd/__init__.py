@@ -5,13 +5,13 @@ benchmark inputs in :mod:`fastdose_amd.synth`; build with ``python -m fastdose_a
 """
 from ._lib import (Beam, Bundle, CSR, FastDoseError, Kernel, Spectrum, Volume, FD_OPT_EXACT_G, FD_OPT_RPL_FP64,
                    FD_OPT_TIMING, LIB_PATH, load)
-from .host import (Context, DoseMat, beam_array, beamlet_flag_init, beamlet_init, beamlets_init, calc_range,
+from .host import (Context, DoseMat, beam_array, bundle_beamlets, beamlet_flag_init, beamlet_init, beamlets_init, calc_range,
                    dense_to_csr, dose_mat_construction, ptv_isocenter, read_dosemat_folder, read_kernel,
                    read_spectrum, write_fluence_map)
 
 __all__ = [
     "Beam", "Bundle", "CSR", "Context", "DoseMat", "FastDoseError", "Kernel", "Spectrum", "Volume",
-    "FD_OPT_EXACT_G", "FD_OPT_RPL_FP64", "FD_OPT_TIMING", "LIB_PATH", "load", "beam_array", "beamlet_flag_init", "beamlet_init",
+    "FD_OPT_EXACT_G", "FD_OPT_RPL_FP64", "FD_OPT_TIMING", "LIB_PATH", "load", "beam_array", "bundle_beamlets", "beamlet_flag_init", "beamlet_init",
     "beamlets_init", "calc_range", "dense_to_csr", "dose_mat_construction", "ptv_isocenter",
     "read_dosemat_folder", "read_kernel", "read_spectrum", "write_fluence_map",
 ]

@@ -102,6 +102,7 @@ SIGNATURES = {
     "fd_dosemat_create": (C.c_int, [C.c_uint64, C.c_uint64, P(vp)]),
     "fd_dosemat_destroy": (C.c_int, [vp]),
     "fd_dosemat_add_beam": (C.c_int, [vp, vp, P(Beam), C.c_int, fp, C.c_float, vp]),
+    "fd_dosemat_add_beams": (C.c_int, [vp, vp, P(Beam), fp, P(C.c_int), C.c_int, C.c_int, C.c_float, vp]),
     "fd_dosemat_add_csr": (C.c_int, [vp, P(CSR), vp]),
     "fd_dosemat_info": (C.c_int, [vp, P(C.c_uint64), P(C.c_uint64), P(C.c_uint64)]),
     "fd_dosemat_matrix_size": (C.c_int, [vp, C.c_uint64, P(C.c_uint64), P(C.c_uint64)]),

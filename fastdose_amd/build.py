@@ -23,6 +23,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = ROOT / "_build"
 LIB = PKG / "libfastdose_hip.so"
+CLI = PKG / "fastdose_imrt"
 ORACLE_DIR = ROOT / "oracle"
 
 HIP_SOURCES = ["api.hip", "kernels_terma.hip", "kernels_cccs.hip", "kernels_resample.hip", "kernels_csr.hip"]
@@ -66,6 +67,13 @@ def build_library(verbose: bool = False) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    # command-line drop-in for the dose mode, linked against the library
+    cli_tmp = CLI.with_name(CLI.name + ".tmp")
+    _run([hipcc, "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(CSRC / "fastdose_imrt.cpp"), f"-L{PKG}",
+          "-lfastdose_hip", "-Wl,-rpath,$ORIGIN", "-o", str(cli_tmp)])
+    os.replace(cli_tmp, CLI)
+    if verbose:
+        print(f"built {CLI}")
     return LIB
 
 

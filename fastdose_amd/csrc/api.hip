@@ -125,6 +125,7 @@ struct fd_ctx {
     DevBuf<uint64_t> cand_off, nnz_off;
     DevBuf<uint64_t> tmp_cols;
     DevBuf<float> tmp_vals;
+    DevBuf<uint64_t> rowoff;          // per-beamlet CSR row offsets of the last chunk
     uint64_t* host_totals = nullptr;  // pinned, 2 entries
     // kernel timers: slot k uses ev[2k] / ev[2k+1]
     hipEvent_t ev[2 * FD_T_COUNT] = {};
@@ -614,6 +615,7 @@ int fd_ctx_destroy(fd_ctx* ctx) {
     ctx->nnz_off.release();
     ctx->tmp_cols.release();
     ctx->tmp_vals.release();
+    ctx->rowoff.release();
     delete ctx;
     return FD_OK;
 }
@@ -856,12 +858,29 @@ int fd_dosemat_destroy(fd_dosemat* m) {
     return FD_OK;
 }
 
-int fd_dosemat_add_beam(fd_ctx* ctx, fd_dosemat* m, const fd_beam* beamlets, int n, const float* fluence,
-                        float extent, void* stream) {
+int fd_dosemat_add_beams(fd_ctx* ctx, fd_dosemat* m, const fd_beam* beamlets, const float* fluence,
+                         const int* counts, int n_beams, int max_chunk, float extent, void* stream) {
     FD_TRY(check_ctx(ctx));
-    FD_TRY(check_beams(beamlets, n));
-    if (!m || (n > 0 && !fluence)) {
+    if (!m || n_beams < 0 || (n_beams > 0 && !counts)) {
         set_error("null argument");
+        return FD_ERR_ARG;
+    }
+    std::vector<uint64_t> first((size_t)n_beams + 1, 0);
+    for (int k = 0; k < n_beams; k++) {
+        if (counts[k] < 0) {
+            set_error("negative beamlet count");
+            return FD_ERR_ARG;
+        }
+        first[k + 1] = first[k] + (uint64_t)counts[k];
+    }
+    const uint64_t total = first[n_beams];
+    if (total > (uint64_t)INT32_MAX) {
+        set_error("too many beamlets");
+        return FD_ERR_ARG;
+    }
+    FD_TRY(check_beams(beamlets, (int)total));
+    if (total > 0 && !fluence) {
+        set_error("null fluence");
         return FD_ERR_ARG;
     }
     const uint64_t nvox = (uint64_t)ctx->vol.dim[0] * ctx->vol.dim[1] * ctx->vol.dim[2];
@@ -869,28 +888,75 @@ int fd_dosemat_add_beam(fd_ctx* ctx, fd_dosemat* m, const fd_beam* beamlets, int
         set_error("dose matrix column count differs from the context's voxel count");
         return FD_ERR_ARG;
     }
+    if (max_chunk <= 0) max_chunk = 8192;
     hipStream_t s = (hipStream_t)stream;
-    const int maxL = max_long_dim(beamlets, n);
-    const uint64_t stride = (uint64_t)maxL * kNpix;
-    uint64_t *off_dst, *col_dst;
-    float* val_dst;
-    if (n == 0) {
-        FD_TRY(append_matrix(m, 0, 0, s, &off_dst, &col_dst, &val_dst));
-        FD_HIP(hipMemsetAsync(off_dst, 0, sizeof(uint64_t), s));
-        return FD_OK;
+    std::vector<uint64_t> host_off, rowoff;
+    int k0 = 0;
+    while (k0 < n_beams) {
+        // whole beams up to max_chunk beamlets (a larger single beam goes alone)
+        int k1 = k0;
+        uint64_t n = 0;
+        while (k1 < n_beams && (k1 == k0 || n + (uint64_t)counts[k1] <= (uint64_t)max_chunk)) n += counts[k1++];
+        const fd_beam* bl = beamlets + first[k0];
+        uint64_t chunk_nnz = 0;
+        if (n > 0) {
+            const int ni = (int)n;
+            const int maxL = max_long_dim(bl, ni);
+            const uint64_t stride = (uint64_t)maxL * kNpix;
+            FD_TRY(ctx->fluence.reserve((size_t)n * kNpix));
+            FD_TRY(ctx->terma.reserve((size_t)n * stride));
+            FD_TRY(ctx->dens_bev.reserve((size_t)n * stride));
+            FD_TRY(ctx->dose_bev.reserve((size_t)n * stride));
+            FD_TRY(ctx->rowoff.reserve((size_t)n + 1));
+            FD_TRY(upload_beams(ctx, bl, ni, stride, s));
+            FD_HIP(hipMemcpyAsync(ctx->fluence.ptr, fluence + first[k0] * kNpix, (size_t)n * kNpix * sizeof(float),
+                                  hipMemcpyHostToDevice, s));
+            FD_TRY(run_terma(ctx, ni, ctx->fluence.ptr, ctx->terma.ptr, ctx->dens_bev.ptr, s));
+            FD_TRY(run_cccs(ctx, ni, maxL, ctx->terma.ptr, ctx->dens_bev.ptr, ctx->dose_bev.ptr, stride, s));
+            FD_TRY(run_resample_values(ctx, ni, ctx->dose_bev.ptr, extent, &chunk_nnz, s));
+            FD_TRY(grow_keep(m->columns, m->used_nnz, m->used_nnz + std::max<uint64_t>(chunk_nnz, 1), s));
+            FD_TRY(grow_keep(m->values, m->used_nnz, m->used_nnz + std::max<uint64_t>(chunk_nnz, 1), s));
+            FD_TRY(run_resample_compact(ctx, ni, ctx->rowoff.ptr, m->columns.ptr + m->used_nnz,
+                                        m->values.ptr + m->used_nnz, s));
+            rowoff.resize(n + 1);
+            FD_HIP(hipMemcpyAsync(rowoff.data(), ctx->rowoff.ptr, (n + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                  s));
+            FD_HIP(hipStreamSynchronize(s));
+        } else {
+            rowoff.assign(1, 0);
+        }
+        // one matrix per beam: offsets re-based to each beam's first row
+        const int nb = k1 - k0;
+        host_off.clear();
+        host_off.reserve(n + nb);
+        const uint64_t off_base = m->used_off;
+        for (int k = k0; k < k1; k++) {
+            const uint64_t a = first[k] - first[k0], c = (uint64_t)counts[k];
+            const uint64_t base = rowoff[a];
+            m->rows.push_back(c);
+            m->nnz.push_back(rowoff[a + c] - base);
+            m->off_off.push_back(off_base + host_off.size());
+            m->nnz_off.push_back(m->used_nnz + base);
+            for (uint64_t j = 0; j <= c; j++) host_off.push_back(rowoff[a + j] - base);
+        }
+        FD_TRY(grow_keep(m->offsets, m->used_off, m->used_off + host_off.size(), s));
+        FD_HIP(hipMemcpyAsync(m->offsets.ptr + m->used_off, host_off.data(), host_off.size() * sizeof(uint64_t),
+                              hipMemcpyHostToDevice, s));
+        FD_HIP(hipStreamSynchronize(s));
+        m->used_off += host_off.size();
+        m->used_nnz += chunk_nnz;
+        k0 = k1;
     }
-    FD_TRY(ctx->fluence.reserve((size_t)n * kNpix));
-    FD_TRY(ctx->terma.reserve((size_t)n * stride));
-    FD_TRY(ctx->dens_bev.reserve((size_t)n * stride));
-    FD_TRY(ctx->dose_bev.reserve((size_t)n * stride));
-    FD_TRY(upload_beams(ctx, beamlets, n, stride, s));
-    FD_HIP(hipMemcpyAsync(ctx->fluence.ptr, fluence, (size_t)n * kNpix * sizeof(float), hipMemcpyHostToDevice, s));
-    FD_TRY(run_terma(ctx, n, ctx->fluence.ptr, ctx->terma.ptr, ctx->dens_bev.ptr, s));
-    FD_TRY(run_cccs(ctx, n, maxL, ctx->terma.ptr, ctx->dens_bev.ptr, ctx->dose_bev.ptr, stride, s));
-    uint64_t nnz = 0;
-    FD_TRY(run_resample_values(ctx, n, ctx->dose_bev.ptr, extent, &nnz, s));
-    FD_TRY(append_matrix(m, (uint64_t)n, nnz, s, &off_dst, &col_dst, &val_dst));
-    return run_resample_compact(ctx, n, off_dst, col_dst, val_dst, s);
+    return FD_OK;
+}
+
+int fd_dosemat_add_beam(fd_ctx* ctx, fd_dosemat* m, const fd_beam* beamlets, int n, const float* fluence,
+                        float extent, void* stream) {
+    if (n < 0) {
+        set_error("negative beamlet count");
+        return FD_ERR_ARG;
+    }
+    return fd_dosemat_add_beams(ctx, m, beamlets, fluence, &n, 1, n, extent, stream);
 }
 
 int fd_dosemat_add_csr(fd_dosemat* m, const fd_csr* csr, void* stream) {

@@ -2,8 +2,8 @@
 //
 // Algorithm (FastDoseLib/src/DoseComp.cu:217-392, maths in KernelGen/derivation.md:94-291).
 // For every forward zenith cone theta and azimuth phi, the 256 BEV rays of a beamlet march
-// through the grid in lockstep: they share one segment sequence (the lattice is shift
-// invariant, lateral coordinates wrap with period 16), each ray adding its lane offset.  For a
+// through the grid in lockstep: they share one segment sequence and each ray sits at the shared
+// lattice point plus its integer lane offset (lateral coordinates wrap with period 16).  For a
 // segment of radiological length p through voxel m of the current slice:
 //     dose[m] += (A/a ((1-g) T_m + g X_A) + B/b ((1-g) T_m + g X_B)) p,   cumu[m] += p
 //     X <- T_m (1 - e^{-a p}) + e^{-a p} X          (same for B, b)
@@ -11,14 +11,17 @@
 // (1-e^{-x})/x, and X reset to 0 when a ray crosses the BEV lateral edge.  At the end of a
 // slice the dose summed over all phi is divided by (cumu of the last phi + 1e-4).
 //
-// Mapping to CDNA4.  Workgroup = (beamlet, forward cone), one wavefront per phi; a wavefront
-// carries all 256 rays of its cone, 4 per lane, with X_A/X_B in registers.  The Terma and
-// density slab of the current slice are staged once in LDS for all azimuths; each wavefront
-// scatters into its own LDS dose slab with LDS float atomics (the per-segment lane -> voxel map
-// is a wrapped shift, so collisions only occur at FP32 rounding edges and are still added), so
-// no two wavefronts touch the same accumulator.  Three barriers per slice.  The march lattice
-// (next point, slice end, wrapped indices, edge crossings) is computed in FP32 with the exact
-// operation order of the CPU oracle; only the physics uses fast transcendentals.
+// Mapping to CDNA4.  Workgroup = (beamlet, forward cone), one wavefront per azimuth.  Because the
+// lane -> voxel map of a segment is a wrapped shift of the shared midpoint voxel, the wavefront
+// keeps everything voxel-indexed in registers: lane l owns voxels (x = l & 15, y = 4 (l >> 4) + r),
+// r = 0..3, and holds their Terma, density, dose and cumulative p for the slice plus the X_A/X_B
+// of the ray currently in each voxel.  When the shared midpoint moves to a neighbouring voxel the
+// X state is rotated with it: along x by a DPP rotation inside each 16-lane row, along y by a
+// register rotation across the four slots plus one cross-row move.  The inner loop therefore
+// touches no LDS memory; LDS only stages the next slice's Terma/density slab (prefetched during
+// the current slice) and the per-azimuth doses for the end-of-slice combine (double-buffered,
+// one barrier per slice).  The march itself (next point, slice end, midpoint voxel, edge
+// crossings) is FP32 with the exact operation order of the CPU oracle.
 //
 // The slice loop follows the reference: the slice processed next is round(z) of the phi-0
 // march, and every azimuth marches from its own position; slices the phi-0 march skips get no
@@ -33,12 +36,28 @@ namespace fdi {
 namespace {
 
 constexpr float kLog2e = 1.4426950408889634f;
+constexpr int kSlots = kRaysPerLane;  // voxels (rows) per lane
 
-__device__ __forceinline__ int wrap16(float m) {
-    // fmodf(m, 16) (exact: m - 16 trunc(m/16)), shifted to [0, 16), floored; 16 wraps to 0
-    float r = m - 16.0f * truncf(m * 0.0625f);
-    r = (r < 0.0f) ? r + 16.0f : r;
-    return ((int)floorf(r)) & 15;
+// Distance to the next voxel boundary along one axis, in units of the direction's parameter
+// (d_nextPoint, DoseComp.cu:76-111): ceil(c + 1e-3) - c going up, floor(c - 1e-3) - c going
+// down, infinite for a zero direction component.  Branch-free.
+__device__ __forceinline__ float axis_step(float c, float d, float inv) {
+    const float up = (ceilf(c + kLargerEps) - c) * inv;
+    const float dn = (floorf(c - kLargerEps) - c) * inv;
+    const float st = d > 0.0f ? up : dn;
+    return d == 0.0f ? INFINITY : st;
+}
+
+// value of the neighbouring lane in the same 16-lane row (DPP row rotation)
+__device__ __forceinline__ float row_from_left(float v) {   // dst[i] = src[i - 1]
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x121, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float row_from_right(float v) {  // dst[i] = src[i + 1]
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x12F, 0xF, 0xF, false));
+}
+// value of lane src_lane
+__device__ __forceinline__ float wave_from(float v, int src_lane) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src_lane << 2, __float_as_int(v)));
 }
 
 template <bool EXACT_G>
@@ -47,19 +66,18 @@ __global__ __launch_bounds__(1024) void cccs_kernel(const BeamletDev* __restrict
                                                     const float* __restrict__ terma,
                                                     const float* __restrict__ dens_bev,
                                                     float* __restrict__ partial, uint64_t stride) {
-    __shared__ float sT[kNpix];
-    __shared__ float sR[kNpix];
-    __shared__ float sC[kNpix];
-    __shared__ float sD[kMaxPhi * kNpix];
-    __shared__ int s_slice;
-    __shared__ int s_next;
+    extern __shared__ float smem[];
+    float* sTR = smem;             // [2 buffers][T | rho][256]
+    float* sC = smem + 4 * kNpix;  // [2 buffers][256]
+    float* sD = smem + 6 * kNpix;  // [2 buffers][n_phi][256]
+    __shared__ int s_next[2];
 
     const int t = blockIdx.x / n;  // forward-cone slot, widest cone first
     const int b = blockIdx.x - t * n;
     const int tid = threadIdx.x;
     const int nthreads = blockDim.x;
     const int n_phi = nthreads >> 6;
-    const int wave = tid >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
     const BeamletDev& B = beams[b];
     const int L = B.L;
@@ -71,68 +89,125 @@ __global__ __launch_bounds__(1024) void cccs_kernel(const BeamletDev* __restrict
 
     // grid direction of (theta, phi_wave): unit vector scaled to voxel units, renormalised
     const float ux = cones->ux[t][wave], uy = cones->uy[t][wave], uz = cones->uz[t][wave];
-    const float vx = ux / B.bs[0], vy = uy / B.bs[1], vz = uz / B.ls;
-    const float rn = 1.0f / sqrtf(vx * vx + vy * vy + vz * vz);
-    const float dx = vx * rn, dy = vy * rn, dz = vz * rn;
+    const float vxd = ux / B.bs[0], vyd = uy / B.bs[1], vzd = uz / B.ls;
+    const float rn = 1.0f / sqrtf(vxd * vxd + vyd * vyd + vzd * vzd);
+    const float dx = vxd * rn, dy = vyd * rn, dz = vzd * rn;
     const float ix_ = 1.0f / dx, iy_ = 1.0f / dy, iz_ = 1.0f / dz;
 
-    // segment-length constants: physical lateral offset = (coord - 8) bs f(z),
-    // f(z) = (z ls + lim_min) / sad
     const float inv_sad = 1.0f / B.sad;
     const float bsx = B.bs[0], bsy = B.bs[1], ls = B.ls, lim_min = B.lim_min;
 
-    const int ix = lane & 15;
-    const int iy0 = lane >> 4;
-    const float fix = (float)ix;
+    // my voxels: x = vx, y = 4 g + r; flat index vbase + 16 r
+    const int vx = lane & 15;
+    const int g = lane >> 4;
+    const int vbase = vx + 64 * g;
+    const int up_lane = (lane + 48) & 63;    // lane (x, g - 1)
+    const int down_lane = (lane + 16) & 63;  // lane (x, g + 1)
 
-    float XA[kRaysPerLane], XB[kRaysPerLane];
+    float XA[kSlots], XB[kSlots];
 #pragma unroll
-    for (int r = 0; r < kRaysPerLane; r++) {
+    for (int r = 0; r < kSlots; r++) {
         XA[r] = 0.0f;
         XB[r] = 0.0f;
     }
     float bxp = 0.5f, byp = 0.5f, bzp = 0.0f;  // march position of this azimuth
+    int mx_prev = 0, my_prev = 0;              // midpoint voxel the X state is aligned to
 
     const float* __restrict__ Tg = terma + B.bev_off;
     const float* __restrict__ Rg = dens_bev + B.bev_off;
     float* __restrict__ P = partial + ((uint64_t)b * n_fwd + t) * stride;
-    float* __restrict__ myD = sD + wave * kNpix;
     const bool last_phi = wave == n_phi - 1;
+    const bool two = nthreads < 2 * kNpix;         // each thread prefetches two slab values
+    const bool prefetch = nthreads >= kNpix;        // smaller workgroups reload each slab
 
-    if (tid == 0) s_slice = 0;
+    // slab of slice 0 into buffer 0 (T at [0, 256), rho at [256, 512))
+    for (int i = tid; i < 2 * kNpix; i += nthreads) sTR[i] = i < kNpix ? Tg[i] : Rg[i - kNpix];
     __syncthreads();
+
+    int s = 0;
+    int buf = 0;
     int written = 0;  // slices [0, written) of P are final
-    while (true) {
-        const int s = s_slice;
-        if (s >= L) break;
-        for (int z = written; z < s; z++)  // slices the phi-0 march stepped over
-            for (int i = tid; i < kNpix; i += nthreads) P[(size_t)z * kNpix + i] = 0.0f;
-        for (int i = tid; i < kNpix; i += nthreads) {
-            sT[i] = Tg[(size_t)s * kNpix + i];
-            sR[i] = Rg[(size_t)s * kNpix + i];
-            sC[i] = 0.0f;
+    while (s < L) {
+        // prefetch the next slice's slab (the usual successor; a skipped slice reloads below)
+        const int sp = s + 1 < L ? s + 1 : s;
+        float pf0 = 0.0f, pf1 = 0.0f;
+        if (prefetch) {
+            const int i0 = tid, i1 = tid + nthreads;
+            if (i0 < 2 * kNpix) pf0 = i0 < kNpix ? Tg[(size_t)sp * kNpix + i0] : Rg[(size_t)sp * kNpix + i0 - kNpix];
+            if (two && i1 < 2 * kNpix)
+                pf1 = i1 < kNpix ? Tg[(size_t)sp * kNpix + i1] : Rg[(size_t)sp * kNpix + i1 - kNpix];
         }
-        for (int i = tid; i < n_phi * kNpix; i += nthreads) sD[i] = 0.0f;
-        __syncthreads();
+        // this slice's Terma and density of my voxels
+        const float* slab = sTR + buf * 2 * kNpix;
+        float T[kSlots], R[kSlots], D[kSlots], Cp[kSlots];
+#pragma unroll
+        for (int r = 0; r < kSlots; r++) {
+            T[r] = slab[vbase + 16 * r];
+            R[r] = slab[kNpix + vbase + 16 * r];
+            D[r] = 0.0f;
+            Cp[r] = 0.0f;
+        }
 
         for (int it = 0; it < kMaxStepsPerSlice; it++) {
             // next lattice point (d_nextPoint, DoseComp.cu:76-111)
-            const float sx = dx > 0.0f ? (ceilf(bxp + kLargerEps) - bxp) * ix_
-                                       : (dx < 0.0f ? (floorf(bxp - kLargerEps) - bxp) * ix_ : INFINITY);
-            const float sy = dy > 0.0f ? (ceilf(byp + kLargerEps) - byp) * iy_
-                                       : (dy < 0.0f ? (floorf(byp - kLargerEps) - byp) * iy_ : INFINITY);
-            const float sz = dz > 0.0f ? (ceilf(bzp + kLargerEps) - bzp) * iz_
-                                       : (dz < 0.0f ? (floorf(bzp - kLargerEps) - bzp) * iz_ : INFINITY);
+            const float sx = axis_step(bxp, dx, ix_);
+            const float sy = axis_step(byp, dy, iy_);
+            const float sz = axis_step(bzp, dz, iz_);
             const float st = fminf(fminf(sx, sy), sz);
             float nx = fmaf(st, dx, bxp), ny = fmaf(st, dy, byp), nz = fmaf(st, dz, bzp);
-            const bool flag = fabsf(nz - roundf(nz)) < kEps;
+            // rint == round here: a tie (.5) is never within 1e-4 of an integer, and the snap is
+            // only applied to values within 1e-4 of one
+            const bool flag = fabsf(nz - rintf(nz)) < kEps;
             if (flag) {
                 nx = fmaf(sz, dx, bxp);
                 ny = fmaf(sz, dy, byp);
-                nz = roundf(fmaf(sz, dz, bzp));
+                nz = rintf(fmaf(sz, dz, bzp));
             }
 
-            // segment length terms shared by all rays
+            // shared midpoint voxel; rotate the X state to follow it
+            const int mx = (int)floorf((bxp + nx) * 0.5f);
+            const int my = (int)floorf((byp + ny) * 0.5f);
+            int ddx = ((mx - mx_prev + 8) & 15) - 8;
+            int ddy = ((my - my_prev + 8) & 15) - 8;
+            mx_prev = mx;
+            my_prev = my;
+            for (; ddx > 0; ddx--)
+#pragma unroll
+                for (int r = 0; r < kSlots; r++) {
+                    XA[r] = row_from_left(XA[r]);
+                    XB[r] = row_from_left(XB[r]);
+                }
+            for (; ddx < 0; ddx++)
+#pragma unroll
+                for (int r = 0; r < kSlots; r++) {
+                    XA[r] = row_from_right(XA[r]);
+                    XB[r] = row_from_right(XB[r]);
+                }
+            for (; ddy > 0; ddy--) {  // new[y] = old[y - 1]
+                const float ta = wave_from(XA[kSlots - 1], up_lane);
+                const float tb = wave_from(XB[kSlots - 1], up_lane);
+#pragma unroll
+                for (int r = kSlots - 1; r > 0; r--) {
+                    XA[r] = XA[r - 1];
+                    XB[r] = XB[r - 1];
+                }
+                XA[0] = ta;
+                XB[0] = tb;
+            }
+            for (; ddy < 0; ddy++) {  // new[y] = old[y + 1]
+                const float ta = wave_from(XA[0], down_lane);
+                const float tb = wave_from(XB[0], down_lane);
+#pragma unroll
+                for (int r = 0; r < kSlots - 1; r++) {
+                    XA[r] = XA[r + 1];
+                    XB[r] = XB[r + 1];
+                }
+                XA[kSlots - 1] = ta;
+                XB[kSlots - 1] = tb;
+            }
+
+            // segment length terms shared by all rays: physical lateral offset of a ray at lattice
+            // coordinate c is (c - 8) bs f(z), f(z) = (z ls + lim_min) / sad (unwrapped)
             const float dzs = (nz - bzp) * ls;
             const float fe = fmaf(nz, ls, lim_min) * inv_sad;
             const float df = dzs * inv_sad;
@@ -140,45 +215,35 @@ __global__ __launch_bounds__(1024) void cccs_kernel(const BeamletDev* __restrict
             const float axc = fmaf(gx, bxp - 8.0f, bsx * (nx - bxp) * fe);
             const float ayc = fmaf(gy, byp - 8.0f, bsy * (ny - byp) * fe);
             const float dz2 = dzs * dzs;
+            const int fbx = (int)floorf(bxp), fnx = (int)floorf(nx);
+            const int fby = (int)floorf(byp), fny = (int)floorf(ny);
 
-            // lane x: wrapped voxel column and edge crossing
-            const float btx = bxp + fix, ntx = nx + fix;
-            const int mx = wrap16((btx + ntx) * 0.5f);
-            const bool cross_x = floorf(btx * 0.0625f) != floorf(ntx * 0.0625f);
-            const float ddx = fmaf(gx, fix, axc);
-            const float dxz2 = fmaf(ddx, ddx, dz2);
-
+            // ray in my column: lane offset, edge crossing, lateral length term
+            const int ix = (vx - mx) & 15;
+            const bool cross_x = ((fbx + ix) >> 4) != ((fnx + ix) >> 4);
+            const float ddxl = fmaf(gx, (float)ix, axc);
+            const float dxz2 = fmaf(ddxl, ddxl, dz2);
 #pragma unroll
-            for (int r = 0; r < kRaysPerLane; r++) {
-                const int iy = iy0 + 4 * r;
-                const float fiy = (float)iy;
-                const float bty = byp + fiy, nty = ny + fiy;
-                const int my = wrap16((bty + nty) * 0.5f);
-                const bool cross = cross_x || (floorf(bty * 0.0625f) != floorf(nty * 0.0625f));
-                const int m = mx + (my << 4);
-                const float Tm = sT[m];
-                const float Rm = sR[m];
-                const float ddy = fmaf(gy, fiy, ayc);
-                const float seg = __builtin_sqrtf(fmaf(ddy, ddy, dxz2));
-                const float p = seg * Rm;
-                // 1 - e^{-ap}: series below 1/128 (exact 0 at p = 0, no cancellation), else exp2
-                const float ap = ca * p, bp = cb * p;
-                const float oma = ap < 0.0078125f ? ap * fmaf(-ap, fmaf(-ap, 1.0f / 6.0f, 0.5f), 1.0f)
-                                                  : 1.0f - exp2f(na * p);
-                const float omb = bp < 0.0078125f ? bp * fmaf(-bp, fmaf(-bp, 1.0f / 6.0f, 0.5f), 1.0f)
-                                                  : 1.0f - exp2f(nb * p);
-                float dose;
+            for (int r = 0; r < kSlots; r++) {
+                const int iy = (4 * g + r - my) & 15;
+                const bool cross = cross_x || (((fby + iy) >> 4) != ((fny + iy) >> 4));
+                const float ddyl = fmaf(gy, (float)iy, ayc);
+                const float p = __builtin_amdgcn_sqrtf(fmaf(ddyl, ddyl, dxz2)) * R[r];
+                // 1 - e^{-ap}, 1 - e^{-bp} (exactly 0 at p = 0)
+                const float oma = 1.0f - __builtin_amdgcn_exp2f(na * p);
+                const float omb = 1.0f - __builtin_amdgcn_exp2f(nb * p);
                 if (EXACT_G) {
-                    const float ga = ap > 0.0f ? oma / ap : 1.0f;
-                    const float gb = bp > 0.0f ? omb / bp : 1.0f;
-                    dose = fmaf(A_a, fmaf(ga, XA[r] - Tm, Tm), B_b * fmaf(gb, XB[r] - Tm, Tm)) * p;
+                    const float ap = ca * p, bp = cb * p;
+                    const float ga = ap > 0.0f ? oma * __builtin_amdgcn_rcpf(ap) : 1.0f;
+                    const float gb = bp > 0.0f ? omb * __builtin_amdgcn_rcpf(bp) : 1.0f;
+                    D[r] = fmaf(fmaf(A_a, fmaf(ga, XA[r] - T[r], T[r]), B_b * fmaf(gb, XB[r] - T[r], T[r])), p,
+                                D[r]);
                 } else {
-                    dose = fmaf(A_a, XA[r], B_b * XB[r]) * p;
+                    D[r] = fmaf(fmaf(A_a, XA[r], B_b * XB[r]), p, D[r]);
                 }
-                atomicAdd(&myD[m], dose);
-                if (last_phi) atomicAdd(&sC[m], p);
-                const float xa = fmaf(oma, Tm - XA[r], XA[r]);
-                const float xb = fmaf(omb, Tm - XB[r], XB[r]);
+                Cp[r] += p;
+                const float xa = fmaf(oma, T[r] - XA[r], XA[r]);
+                const float xb = fmaf(omb, T[r] - XB[r], XB[r]);
                 XA[r] = cross ? 0.0f : xa;
                 XB[r] = cross ? 0.0f : xb;
             }
@@ -187,16 +252,43 @@ __global__ __launch_bounds__(1024) void cccs_kernel(const BeamletDev* __restrict
             bzp = nz;
             if (flag) break;
         }
-        if (tid == 0) s_next = max((int)roundf(bzp), s + 1);  // always advances
+
+        // publish this azimuth's slice dose, the next slab and the next slice index
+        float* myD = sD + (buf * n_phi + wave) * kNpix;
+#pragma unroll
+        for (int r = 0; r < kSlots; r++) myD[vbase + 16 * r] = D[r];
+        if (last_phi) {
+#pragma unroll
+            for (int r = 0; r < kSlots; r++) sC[buf * kNpix + vbase + 16 * r] = Cp[r];
+        }
+        {
+            float* nslab = sTR + (buf ^ 1) * 2 * kNpix;
+            const int i0 = tid, i1 = tid + nthreads;
+            if (i0 < 2 * kNpix) nslab[i0] = pf0;
+            if (two && i1 < 2 * kNpix) nslab[i1] = pf1;
+        }
+        if (tid == 0) s_next[buf] = max((int)rintf(bzp), s + 1);  // always advances
         __syncthreads();
+        const int sn = s_next[buf];
+        // combine slice s: sum over azimuths / (cumulative p of the last azimuth + 1e-4)
         for (int i = tid; i < kNpix; i += nthreads) {
             float sum = 0.0f;
-            for (int f = 0; f < n_phi; f++) sum += sD[f * kNpix + i];
-            P[(size_t)s * kNpix + i] = sum / (sC[i] + kEps);
+            for (int f = 0; f < n_phi; f++) sum += sD[(buf * n_phi + f) * kNpix + i];
+            P[(size_t)s * kNpix + i] = sum / (sC[buf * kNpix + i] + kEps);
         }
-        written = s + 1;
-        if (tid == 0) s_slice = s_next;
-        __syncthreads();
+        for (int z = s + 1; z < sn && z < L; z++)  // slices the phi-0 march stepped over
+            for (int i = tid; i < kNpix; i += nthreads) P[(size_t)z * kNpix + i] = 0.0f;
+        written = sn < L ? sn : L;
+        if ((sn != sp || !prefetch) && sn < L) {
+            // the phi-0 march skipped a slice (or no prefetch): load the slab, then barrier
+            __syncthreads();
+            float* nslab = sTR + (buf ^ 1) * 2 * kNpix;
+            for (int i = tid; i < 2 * kNpix; i += nthreads)
+                nslab[i] = i < kNpix ? Tg[(size_t)sn * kNpix + i] : Rg[(size_t)sn * kNpix + i - kNpix];
+            __syncthreads();
+        }
+        s = sn;
+        buf ^= 1;
     }
     for (int z = written; z < L; z++)
         for (int i = tid; i < kNpix; i += nthreads) P[(size_t)z * kNpix + i] = 0.0f;
@@ -226,11 +318,12 @@ hipError_t launch_cccs(const BeamletDev* beams, int n, int max_L, const ConeTabl
     if (ev) (void)hipEventRecord(ev[0], stream);
     if (n_fwd > 0) {
         dim3 grid((unsigned)(n * n_fwd)), block(n_phi * kWave);
+        const size_t shm = (size_t)(6 + 2 * n_phi) * kNpix * sizeof(float);
         if (exact_g)
-            hipLaunchKernelGGL(cccs_kernel<true>, grid, block, 0, stream, beams, n, cones_dev, n_fwd, terma,
+            hipLaunchKernelGGL(cccs_kernel<true>, grid, block, shm, stream, beams, n, cones_dev, n_fwd, terma,
                                dens_bev, partial, stride);
         else
-            hipLaunchKernelGGL(cccs_kernel<false>, grid, block, 0, stream, beams, n, cones_dev, n_fwd, terma,
+            hipLaunchKernelGGL(cccs_kernel<false>, grid, block, shm, stream, beams, n, cones_dev, n_fwd, terma,
                                dens_bev, partial, stride);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;

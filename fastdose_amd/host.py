@@ -235,6 +235,19 @@ class DoseMat:
             raise ValueError("fluence must hold 256 floats per beamlet")
         call("fd_dosemat_add_beam", ctx.handle, self._h, arr, len(arr), _f32(flu), extent, _stream_arg(stream))
 
+    def add_beams(self, ctx: Context, beamlets, fluence: np.ndarray, counts, max_chunk: int = 0,
+                  extent: float = 2.0, stream=None) -> None:
+        """Batched: all beams' beamlets through one launch per stage (per chunk), one matrix per beam."""
+        arr = beam_array(beamlets)
+        cnt = np.ascontiguousarray(counts, dtype=np.int32)
+        if int(cnt.sum()) != len(arr):
+            raise ValueError("counts must add up to the number of beamlets")
+        flu = np.ascontiguousarray(fluence, dtype=np.float32)
+        if flu.size != len(arr) * NPIX:
+            raise ValueError("fluence must hold 256 floats per beamlet")
+        call("fd_dosemat_add_beams", ctx.handle, self._h, arr, _f32(flu), cnt.ctypes.data_as(C.POINTER(C.c_int)),
+             len(cnt), max_chunk, extent, _stream_arg(stream))
+
     def info(self):
         a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
         call("fd_dosemat_info", self._h, C.byref(a), C.byref(b), C.byref(c))
@@ -283,11 +296,28 @@ def read_dosemat_folder(folder):
     return mats
 
 
+def bundle_beamlets(bundles, flags_list, vol: Volume):
+    """Beamlets of every bundle, concatenated: (Beam array, fluence (n, 256), per-beam counts)."""
+    all_beams, flus, counts = [], [], []
+    for bundle, flags in zip(bundles, flags_list):
+        beams, flu = beamlets_init(bundle, flags, vol)
+        all_beams.extend(beams)
+        flus.append(flu)
+        counts.append(len(beams))
+    flu = np.concatenate(flus, 0) if flus else np.zeros((0, NPIX), np.float32)
+    return beam_array(all_beams), flu, np.array(counts, dtype=np.int32)
+
+
 def dose_mat_construction(ctx: Context, bundles, flags_list, extent: float = 2.0, est_nnz: int = 0,
-                          stream=None) -> DoseMat:
-    """DoseMatConstruction (IMRTDoseMat.cu:14-301): one matrix per beam, beams in order."""
+                          stream=None, batched: bool = False, max_chunk: int = 0) -> DoseMat:
+    """DoseMatConstruction (IMRTDoseMat.cu:14-301): one matrix per beam, beams in order.  batched=True
+    sends all beams through the kernels together (fd_dosemat_add_beams)."""
     nvox = ctx.vol.dim[0] * ctx.vol.dim[1] * ctx.vol.dim[2]
     dm = DoseMat(nvox, est_nnz)
+    if batched:
+        beams, flu, counts = bundle_beamlets(bundles, flags_list, ctx.vol)
+        dm.add_beams(ctx, beams, flu, counts, max_chunk, extent, stream)
+        return dm
     for bundle, flags in zip(bundles, flags_list):
         beams, flu = beamlets_init(bundle, flags, ctx.vol)
         dm.add_beam(ctx, beams, flu, extent, stream)

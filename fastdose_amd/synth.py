@@ -153,3 +153,95 @@ def small_config(seed: int = 7, dim=(24, 28, 20), fluence_dim: int = 2, angles=(
     b = make_bundle(iso, angles, fluence_dim=fluence_dim)
     return Config(f"small-{seed}", vol, density, synthetic_spectrum(1000 + seed), synthetic_kernel(1000 + seed), [b],
                   [np.ones(fluence_dim * fluence_dim, dtype=np.uint8)])
+
+
+def direction_to_varian_iec(direction):
+    """(gantry, couch) in rad of a source->isocenter direction (dockerScripts/HeadNeck/scripts.py:47-62)."""
+    eps = 1e-4
+    cos_g = float(direction[1])
+    sin_g = math.sqrt(max(0.0, 1.0 - cos_g * cos_g))
+    gantry = math.acos(max(-1.0, min(1.0, cos_g)))
+    if sin_g < eps:
+        minus_couch = 0.0
+    else:
+        sin_mc = direction[2] / sin_g
+        cos_mc = max(-1.0, min(1.0, -direction[0] / sin_g))
+        minus_couch = math.acos(cos_mc)
+        if sin_mc < 0:
+            minus_couch = -minus_couch
+    return gantry, -minus_couch
+
+
+def fibonacci_sphere(n: int) -> np.ndarray:
+    """n unit vectors spread evenly over the sphere (golden-angle spiral)."""
+    i = np.arange(n) + 0.5
+    z = 1.0 - 2.0 * i / n
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    phi = math.pi * (3.0 - math.sqrt(5.0)) * i
+    return np.stack([r * np.cos(phi), r * np.sin(phi), z], 1)
+
+
+def body_density(dim, seed: int, sigma: float = 4.0):
+    """Gaussian-smoothed seeded noise mapped to [0.2, 1.8] g/cc inside an ellipsoidal body, 0.0012
+    outside.  Returns (density (Z, Y, X) float32, body mask uint8)."""
+    from scipy.ndimage import gaussian_filter
+
+    rng = np.random.default_rng(seed)
+    noise = gaussian_filter(rng.standard_normal(dim[::-1]).astype(np.float32), sigma=sigma)
+    z, y, x = np.meshgrid(*(np.arange(d, dtype=np.float32) + 0.5 for d in dim[::-1]), indexing="ij")
+    cx, cy, cz = (d / 2.0 for d in dim)
+    body = ((x - cx) / (0.45 * dim[0])) ** 2 + ((y - cy) / (0.40 * dim[1])) ** 2 + \
+        ((z - cz) / (0.47 * dim[2])) ** 2 <= 1.0
+    inside = noise[body]
+    lo, hi = float(inside.min()), float(inside.max())
+    rho = np.full(dim[::-1], 0.0012, dtype=np.float32)
+    rho[body] = 0.2 + 1.6 * (inside - lo) / (hi - lo)
+    return rho, body.astype(np.uint8)
+
+
+def ptv_segments(dim, center, radii, n_seg: int = 4):
+    """An ellipsoidal PTV (voxel units) split into n_seg slabs along x: list of uint8 masks."""
+    z, y, x = np.meshgrid(*(np.arange(d, dtype=np.float32) + 0.5 for d in dim[::-1]), indexing="ij")
+    ell = ((x - center[0]) / radii[0]) ** 2 + ((y - center[1]) / radii[1]) ** 2 + \
+        ((z - center[2]) / radii[2]) ** 2 <= 1.0
+    edges = np.linspace(center[0] - radii[0], center[0] + radii[0], n_seg + 1)
+    segs = []
+    for k in range(n_seg):
+        segs.append((ell & (x >= edges[k]) & (x < edges[k + 1] + (k == n_seg - 1))).astype(np.uint8))
+    return segs, ell.astype(np.uint8)
+
+
+def fourpi_config(name, dim, seed, n_beams, ptv_radii, fluence_dim=20, with_flags=True) -> Config:
+    """4pi plan: body phantom, PTV in 4 segments with one isocenter each (segment centroids), n_beams
+    Fibonacci-sphere beams assigned round-robin to segments, beamlets kept where they intersect their
+    segment (beamletFlagInit, IMRTBeamBundle.cu:130-316)."""
+    from . import host
+
+    vol = make_volume(dim)
+    density, body = body_density(dim, seed)
+    center = [d / 2.0 for d in dim]
+    segs, ptv = ptv_segments(dim, center, ptv_radii)
+    isos = [host.ptv_isocenter(m, vol) for m in segs]
+    bundles, flags, seg_of = [], [], []
+    for k, d in enumerate(fibonacci_sphere(n_beams)):
+        gantry, couch = direction_to_varian_iec(d)
+        deg = (round(math.degrees(gantry), 4), round(math.degrees(couch), 4), 0.0)  # beamlist text precision
+        s = k % len(segs)
+        b = make_bundle(isos[s], deg, fluence_dim=fluence_dim)
+        bundles.append(b)
+        seg_of.append(s)
+        flags.append(host.beamlet_flag_init(b, segs[s], vol) if with_flags else None)
+    cfg = Config(name, vol, density, synthetic_spectrum(seed), synthetic_kernel(seed), bundles, flags, ptv=ptv)
+    cfg.notes = {"segments": segs, "segment_of_beam": seg_of, "isocenters": [list(map(float, i)) for i in isos],
+                 "body": body}
+    return cfg
+
+
+def config3(n_beams: int = 180) -> Config:
+    """Batched 4pi: 200x200x160 head-and-neck-sized body, 4 isocenters, 180 beams, 20x20 fluence."""
+    return fourpi_config("batched-4pi-200x200x160", (200, 200, 160), 1003, n_beams, (22.0, 15.0, 13.0))
+
+
+def config5_candidates(n_beams: int = 1160) -> Config:
+    """Serving: 220x220x150 pancreas-sized body with 1160 candidate 4pi beams."""
+    return fourpi_config("serving-4pi-220x220x150", (220, 220, 150), 1005, n_beams, (26.0, 18.0, 15.0))

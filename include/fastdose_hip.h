@@ -215,6 +215,13 @@ int fd_dosemat_destroy(fd_dosemat* m);
  * fd_beamlet_init writes them. */
 int fd_dosemat_add_beam(fd_ctx* ctx, fd_dosemat* m, const fd_beam* beamlets, int n, const float* fluence,
                         float extent, void* stream);
+/* Batched DoseMatConstruction: n_beams beams whose beamlets are concatenated in `beamlets`
+ * (counts[k] beamlets for beam k, fluence n_total*npix host floats) go through the kernels
+ * together -- all beams of a chunk in one launch per stage, chunks sized to `max_chunk` beamlets
+ * (0: library default) -- and are appended as n_beams matrices, in order.  Each beam's matrix is
+ * identical to what fd_dosemat_add_beam gives for it alone. */
+int fd_dosemat_add_beams(fd_ctx* ctx, fd_dosemat* m, const fd_beam* beamlets, const float* fluence,
+                         const int* counts, int n_beams, int max_chunk, float extent, void* stream);
 /* Appends an existing device CSR (copied). */
 int fd_dosemat_add_csr(fd_dosemat* m, const fd_csr* csr, void* stream);
 int fd_dosemat_info(const fd_dosemat* m, uint64_t* n_matrices, uint64_t* total_rows, uint64_t* total_nnz);

@@ -34,9 +34,13 @@
 //   Q3  dose-matrix resampling is point sampling (IMRTgeom.cu:294-300).
 //   Q6  supersamples with u or w == 16, or beyond the beamlet's own long_dim, read 0.
 //   Q7  lateral periodic wrap + X reset at BEV edge crossings; forward cones only.
-//   Q8  race-free intent: every lane steps from the same pre-step base; resets precede adds.
+//   Q8  race-free intent: every lane steps from the same pre-step base; resets precede adds.  A
+//       ray is exactly the shared lattice point plus its integer lane offset (no per-lane FP32
+//       rounding of the offset sum), so the lane -> voxel map of a segment is a wrapped shift.
 //   Q9  per slice, dose summed over all phi is normalised by the LAST phi's cumulative p.
-//   Q14 a wrapped lateral index that rounds to fmap (16) wraps to 0.
+//   Q14 the wrapped lateral index is (floor(shared midpoint) + lane offset) mod 16 on the integer
+//       (the reference's per-lane fmodf then +16 in FP32 can round a value just below an integer
+//       up to it, or to 16 itself).
 //   Q15 slice loop driven by phi 0's position (sliceIdx = round(base[0].z), DoseComp.cu:277);
 //       a march that skips a slice boundary (ceil(z + 1e-3) rule) is kept as is.
 //   Q16 segment length uses unwrapped lateral coordinates in the divergence factor.
@@ -437,12 +441,38 @@ inline f3 next_point(f3 cur, f3 d, f3 inv, bool* flag) {
     return r;
 }
 
-// d_calcCoords (DoseComp.cu:147-164) for one lateral axis; Q14 wraps a result of n to 0.
+// d_calcCoords (DoseComp.cu:147-164) for one lateral axis: periodic wrap of the FP32 midpoint,
+// taken as floor(m) mod n on the integer (Q14: the reference's fmodf + n can round a value just
+// below an integer up to it, or to n itself).
 inline int wrap_index(float m, float n) {
-    float r = std::fmod(m, n);
-    r = (r < 0) ? r + n : r;
-    int i = (int)floorf(r);
-    return (i >= (int)n) ? i - (int)n : i;
+    long long i = (long long)floorf(m);
+    long long k = (long long)n;
+    long long r = i % k;
+    return (int)(r < 0 ? r + k : r);
+}
+
+inline long long mod_ll(long long a, long long n) {
+    long long r = a % n;
+    return r < 0 ? r + n : r;
+}
+inline long long floordiv_ll(long long a, long long n) { return (a - mod_ll(a, n)) / n; }
+
+// d_calcLineSeg (DoseComp.cu:114-144) in FP64: physical distance of two lattice points
+// (lateral offsets scaled by the divergence factor (z ls + lim_min) / sad, unwrapped -- Q16).
+double line_seg64(const double* s, const double* e, const orc_beam* bm) {
+    double hx = bm->fmap_size[0] * 0.5, hy = bm->fmap_size[1] * 0.5;
+    double bsx = bm->beamlet_size[0], bsy = bm->beamlet_size[1], ls = bm->long_spacing;
+    double lm = bm->lim_min, sad = bm->sad;
+    double sx = (s[0] - hx) * bsx, sy = (s[1] - hy) * bsy, sz = s[2] * ls;
+    double f = (sz + lm) / sad;
+    sx *= f;
+    sy *= f;
+    double ex = (e[0] - hx) * bsx, ey = (e[1] - hy) * bsy, ez = e[2] * ls;
+    f = (ez + lm) / sad;
+    ex *= f;
+    ey *= f;
+    double dx = ex - sx, dy = ey - sy, dz = ez - sz;
+    return std::sqrt(dx * dx + dy * dy + dz * dz);
 }
 
 // Per-beamlet, per-cone grid direction (DoseComp.cu:250-259): unit vector of (theta, phi)
@@ -497,17 +527,8 @@ int orc_next_point(const float* cur, const float* d, const float* inv, float* ou
 int orc_wrap_index(float m, float n) { return wrap_index(m, n); }
 // d_calcLineSeg (DoseComp.cu:114-144) in FP64 from FP32 endpoints
 double orc_line_seg(const float* s, const float* e, const orc_beam* bm) {
-    double hx = bm->fmap_size[0] * 0.5, hy = bm->fmap_size[1] * 0.5;
-    double bsx = bm->beamlet_size[0], bsy = bm->beamlet_size[1], ls = bm->long_spacing;
-    double lm = bm->lim_min, sad = bm->sad;
-    double sx = (s[0] - hx) * bsx, sy = (s[1] - hy) * bsy, sz = s[2] * ls;
-    double f = (sz + lm) / sad;
-    sx *= f; sy *= f;
-    double ex = (e[0] - hx) * bsx, ey = (e[1] - hy) * bsy, ez = e[2] * ls;
-    f = (ez + lm) / sad;
-    ex *= f; ey *= f;
-    double dx = ex - sx, dy = ey - sy, dz = ez - sz;
-    return std::sqrt(dx * dx + dy * dy + dz * dz);
+    const double sd[3] = {s[0], s[1], s[2]}, ed[3] = {e[0], e[1], e[2]};
+    return line_seg64(sd, ed, bm);
 }
 
 // IMRTBeamBundle.cu:36-65 -- PTV centroid (double) in voxel units then cm, returned as FP32
@@ -621,7 +642,6 @@ int64_t orc_cccs(const orc_beam* bm, const double* terma, const double* dens, co
                  int flags, double* dose) {
     const int fx = bm->fmap_size[0], fy = bm->fmap_size[1], npix = fx * fy;
     const int L = bm->long_dim;
-    const float nfx = (float)fx, nfy = (float)fy;
     std::fill(dose, dose + (size_t)L * npix, 0.0);
     const bool exact_g = (flags & ORC_EXACT_G) != 0;
     const double eps_dose = (double)kEps;
@@ -657,14 +677,21 @@ int64_t orc_cccs(const orc_beam* bm, const double* terma, const double* dens, co
                 while (true) {
                     bool flag;
                     f3 np = next_point(base[f], dir[f], inv[f], &flag);
+                    // Q8: every ray sits exactly at the shared lattice point plus its integer lane
+                    // offset, so its midpoint voxel is the shared midpoint voxel shifted by the
+                    // offset and wrapped (Q14), and a BEV-edge crossing is a change of the 16-block
+                    // of the integer floor along the segment.
+                    const long long Mx = (long long)floorf((base[f].x + np.x) * 0.5f);
+                    const long long My = (long long)floorf((base[f].y + np.y) * 0.5f);
+                    const long long Fbx = (long long)floorf(base[f].x), Fnx = (long long)floorf(np.x);
+                    const long long Fby = (long long)floorf(base[f].y), Fny = (long long)floorf(np.y);
                     for (int p = 0; p < npix; p++) {
                         int ix = p % fx, iy = p / fx;
-                        float bt[3] = {base[f].x + ix, base[f].y + iy, base[f].z};
-                        float nt[3] = {np.x + ix, np.y + iy, np.z};
-                        float mxf = (bt[0] + nt[0]) * 0.5f, myf = (bt[1] + nt[1]) * 0.5f;
-                        int m = wrap_index(mxf, nfx) + fx * wrap_index(myf, nfy);
+                        int m = (int)mod_ll(Mx + ix, fx) + fx * (int)mod_ll(My + iy, fy);
                         double Tm = T[m], Rm = R[m];
-                        double p_i = orc_line_seg(bt, nt, bm) * Rm;
+                        const double bt[3] = {(double)base[f].x + ix, (double)base[f].y + iy, (double)base[f].z};
+                        const double nt[3] = {(double)np.x + ix, (double)np.y + iy, (double)np.z};
+                        double p_i = line_seg64(bt, nt, bm) * Rm;
                         double xav = xa[p], xbv = xb[p];
                         double ga = gfun(a * p_i), gb = gfun(b * p_i);
                         double segd = A / a * ((1 - ga) * Tm + ga * xav) + B / b * ((1 - gb) * Tm + gb * xbv);
@@ -673,7 +700,8 @@ int64_t orc_cccs(const orc_beam* bm, const double* terma, const double* dens, co
                         double ea = std::exp(-a * p_i), eb = std::exp(-b * p_i);
                         double nxa = Tm * (1 - ea) + ea * xav;
                         double nxb = Tm * (1 - eb) + eb * xbv;
-                        if (floorf(bt[0] / nfx) != floorf(nt[0] / nfx) || floorf(bt[1] / nfy) != floorf(nt[1] / nfy)) {
+                        if (floordiv_ll(Fbx + ix, fx) != floordiv_ll(Fnx + ix, fx) ||
+                            floordiv_ll(Fby + iy, fy) != floordiv_ll(Fny + iy, fy)) {
                             nxa = 0.0;
                             nxb = 0.0;
                         }

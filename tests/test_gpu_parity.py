@@ -196,6 +196,26 @@ def test_dosemat_files_roundtrip(torch_cuda, tmp_path):
         assert np.array_equal(col.astype(np.int64), g["cols"]) and np.array_equal(val, g["vals"])
 
 
+def test_batched_beams_match_per_beam(torch_cuda):
+    """fd_dosemat_add_beams: beams batched into chunks give each beam's per-beam matrix, bit for bit."""
+    cfg = synth.small_config(seed=12, dim=(28, 26, 24), fluence_dim=3)
+    iso = cfg.bundles[0].isocenter
+    for ang in ((200.0, 10.0, 5.0), (75.0, -40.0, 0.0), (310.0, 80.0, 30.0)):
+        cfg.bundles.append(synth.make_bundle(iso, ang, fluence_dim=3))
+        cfg.flags.append(np.ones(9, dtype=np.uint8))
+    cfg.flags[2] = np.zeros(9, dtype=np.uint8)  # a beam with no active beamlet
+    with fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel) as ctx:
+        ref = fd.dose_mat_construction(ctx, cfg.bundles, cfg.flags, cfg.extent)
+        for chunk in (0, 7, 1):
+            log(f"batched chunk={chunk}")
+            dm = fd.dose_mat_construction(ctx, cfg.bundles, cfg.flags, cfg.extent, batched=True, max_chunk=chunk)
+            assert dm.info() == ref.info()
+            for k in range(len(cfg.bundles)):
+                a, b = ref.matrix(k), dm.matrix(k)
+                assert all(np.array_equal(x, y) for x, y in zip(a, b)), (chunk, k)
+        assert ref.matrix(2)[0].tolist() == [0]
+
+
 def test_error_paths(torch_cuda):
     torch = torch_cuda
     cfg = synth.small_config(seed=11, dim=(16, 16, 16), fluence_dim=1)

@@ -145,13 +145,17 @@ def test_next_point_properties():
             assert min(abs(nxt[0] - round(nxt[0])), abs(nxt[1] - round(nxt[1]))) < 1e-3
 
 
-def test_wrap_index_matches_fmod():
+def test_wrap_index_is_periodic_floor():
+    # d_calcCoords (DoseComp.cu:147-164): fmod + shift to [0, 16), floored; away from FP32
+    # rounding edges this is floor(m) mod 16
     rng = np.random.default_rng(5)
     for m in np.concatenate([rng.uniform(-3000, 3000, 4000), np.arange(-64, 64, 0.5)]).astype(np.float32):
         r = math.fmod(float(m), 16.0)
         r = r + 16.0 if r < 0 else r
-        want = int(math.floor(np.float32(r))) % 16
-        assert orc.lib().orc_wrap_index(m, 16.0) == want
+        assert orc.lib().orc_wrap_index(m, 16.0) == int(math.floor(r)) % 16
+    # the rounding edge: just below a multiple of 16 stays at 15 (fmodf + 16 would give 16)
+    assert orc.lib().orc_wrap_index(np.float32(-1e-7), 16.0) == 15
+    assert orc.lib().orc_wrap_index(np.float32(-32.0), 16.0) == 0
 
 
 # ---------------------------------------------------------------- table readers (spectrum.cu, kernel.cu)
