@@ -22,6 +22,10 @@ FD_ERR_UNSUPPORTED = -6
 FD_OPT_EXACT_G = 1
 FD_OPT_RPL_FP64 = 2
 FD_OPT_TIMING = 4
+FD_OPT_RPL_FP64_ALL = 8
+FD_VAL_F32 = 0
+FD_VAL_BF16 = 1
+FD_VAL_FP16 = 2
 
 TIMING_SLOTS = ("terma", "cccs", "cone_sum", "resample_count", "resample_values", "resample_compact")
 
@@ -73,6 +77,18 @@ class CSR(C.Structure):
                 ("offsets", C.c_void_p), ("columns", C.c_void_p), ("values", C.c_void_p)]
 
 
+class ServeOpts(C.Structure):
+    _fields_ = [("n_slots", C.c_int), ("max_batch_beamlets", C.c_int), ("hbm_budget", C.c_uint64),
+                ("extent", C.c_float), ("options", C.c_int), ("threshold", C.c_float)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("requests_done", C.c_uint64), ("beams_done", C.c_uint64), ("beamlets_done", C.c_uint64),
+                ("batches", C.c_uint64), ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
+                ("max_ms", C.c_double), ("first_submit_s", C.c_double), ("last_done_s", C.c_double),
+                ("peak_inflight_bytes", C.c_uint64), ("peak_inflight_batches", C.c_int)]
+
+
 P = C.POINTER
 vp = C.c_void_p
 u8p = P(C.c_uint8)
@@ -87,6 +103,7 @@ SIGNATURES = {
     "fd_beamlets_init": (C.c_int, [P(Bundle), u8p, P(Volume), P(Beam), fp, P(C.c_int)]),
     "fd_ptv_isocenter": (C.c_int, [u8p, P(Volume), fp]),
     "fd_beamlet_flags_host": (C.c_int, [P(Bundle), u8p, P(Volume), u8p]),
+    "fd_beamlet_flags": (C.c_int, [vp, vp, P(Bundle), C.c_int, u8p, vp]),
     "fd_read_spectrum": (C.c_int, [C.c_char_p, P(Spectrum)]),
     "fd_read_kernel": (C.c_int, [C.c_char_p, C.c_int, P(Kernel)]),
     "fd_ctx_create": (C.c_int, [C.c_int, P(Volume), fp, P(Spectrum), P(Kernel), P(vp)]),
@@ -94,6 +111,7 @@ SIGNATURES = {
     "fd_ctx_set_options": (C.c_int, [vp, C.c_int]),
     "fd_ctx_density": (C.c_int, [vp, P(vp)]),
     "fd_ctx_timings": (C.c_int, [vp, fp, C.c_int]),
+    "fd_ctx_set_threshold": (C.c_int, [vp, C.c_float]),
     "fd_terma": (C.c_int, [vp, P(Beam), C.c_int, vp, vp, vp, C.c_uint64, vp]),
     "fd_cccs": (C.c_int, [vp, P(Beam), C.c_int, vp, vp, vp, C.c_uint64, vp]),
     "fd_resample_csr": (C.c_int, [vp, P(Beam), C.c_int, vp, C.c_uint64, C.c_float, P(CSR), vp]),
@@ -110,6 +128,20 @@ SIGNATURES = {
     "fd_dosemat_tofile": (C.c_int, [vp, C.c_char_p]),
     "fd_dosemat_fromfile": (C.c_int, [vp, C.c_char_p]),
     "fd_write_fluence_map": (C.c_int, [C.c_char_p, u8p, C.c_uint64, C.c_uint64]),
+    "fd_dosemat_pack": (C.c_int, [vp, C.c_int, vp]),
+    "fd_dosemat_copy_packed": (C.c_int, [vp, fp, P(C.c_uint16)]),
+    "fd_spmm_create": (C.c_int, [vp, C.c_int, P(vp), vp]),
+    "fd_spmm_info": (C.c_int, [vp, P(C.c_uint64), P(C.c_uint64), P(C.c_uint64)]),
+    "fd_spmm_run": (C.c_int, [vp, vp, C.c_int, vp, vp]),
+    "fd_spmm_destroy": (C.c_int, [vp]),
+    "fd_serve_start": (C.c_int, [vp, P(ServeOpts), P(vp)]),
+    "fd_serve_submit": (C.c_int, [vp, P(Beam), fp, P(C.c_int), C.c_int, P(C.c_uint64)]),
+    "fd_serve_poll": (C.c_int, [vp, C.c_uint64, P(C.c_int)]),
+    "fd_serve_wait": (C.c_int, [vp, C.c_uint64, C.c_double]),
+    "fd_serve_fetch": (C.c_int, [vp, C.c_uint64, vp]),
+    "fd_serve_stats_get": (C.c_int, [vp, P(ServeStats)]),
+    "fd_serve_stats_reset": (C.c_int, [vp]),
+    "fd_serve_stop": (C.c_int, [vp]),
 }
 
 _lib = None

@@ -26,7 +26,7 @@ LIB = PKG / "libfastdose_hip.so"
 CLI = PKG / "fastdose_imrt"
 ORACLE_DIR = ROOT / "oracle"
 
-HIP_SOURCES = ["api.hip", "kernels_terma.hip", "kernels_cccs.hip", "kernels_resample.hip", "kernels_csr.hip"]
+HIP_SOURCES = ["api.hip", "engine.hip", "serve.hip", "kernels_terma.hip", "kernels_cccs.hip", "kernels_resample.hip", "kernels_csr.hip", "kernels_flags.hip", "kernels_spmm.hip", "spmm.hip"]
 CXX_SOURCES = ["geometry.cpp"]
 ARCH = os.environ.get("FASTDOSE_OFFLOAD_ARCH", "gfx950")
 

@@ -18,6 +18,7 @@
 #include <vector>
 #include <sys/stat.h>
 
+#include "engine.h"
 #include "fd_internal.h"
 #include "geometry.h"
 #include "kernels.h"
@@ -30,125 +31,9 @@ thread_local std::string g_error;
 
 void set_error(const std::string& msg) { g_error = msg; }
 
-namespace {
-
-#define FD_HIP(call)                                                                        \
-    do {                                                                                    \
-        hipError_t e_ = (call);                                                             \
-        if (e_ != hipSuccess) {                                                             \
-            set_error(std::string(#call) + ": " + hipGetErrorString(e_));                   \
-            return FD_ERR_HIP;                                                              \
-        }                                                                                   \
-    } while (0)
-
-#define FD_TRY(call)              \
-    do {                                  \
-        int r_ = (call);                  \
-        if (r_ != FD_OK) return r_;       \
-    } while (0)
-
-// Growable device allocation.
-template <typename T>
-struct DevBuf {
-    T* ptr = nullptr;
-    size_t cap = 0;  // elements
-    int reserve(size_t n) {
-        if (n <= cap) return FD_OK;
-        size_t want = std::max(n, cap + cap / 2);
-        if (ptr) {
-            hipError_t e = hipFree(ptr);
-            ptr = nullptr;
-            cap = 0;
-            if (e != hipSuccess) {
-                set_error(std::string("hipFree: ") + hipGetErrorString(e));
-                return FD_ERR_HIP;
-            }
-        }
-        hipError_t e = hipMalloc((void**)&ptr, want * sizeof(T));
-        if (e != hipSuccess) {
-            ptr = nullptr;
-            set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
-            return FD_ERR_HIP;
-        }
-        cap = want;
-        return FD_OK;
-    }
-    void release() {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        cap = 0;
-    }
-};
-
-// Growable device buffer that keeps its contents (used by the dose-matrix ensemble; doubles
-// like MatCSREnsemble::addMat, IMRTSparseMatEns.cu:125-149).
-template <typename T>
-int grow_keep(DevBuf<T>& b, size_t used, size_t need, hipStream_t stream) {
-    if (need <= b.cap) return FD_OK;
-    size_t want = std::max<size_t>(need, b.cap ? b.cap * 2 : 1024);
-    T* np = nullptr;
-    FD_HIP(hipMalloc((void**)&np, want * sizeof(T)));
-    if (used && b.ptr) FD_HIP(hipMemcpyAsync(np, b.ptr, used * sizeof(T), hipMemcpyDeviceToDevice, stream));
-    FD_HIP(hipStreamSynchronize(stream));
-    if (b.ptr) FD_HIP(hipFree(b.ptr));
-    b.ptr = np;
-    b.cap = want;
-    return FD_OK;
-}
-
-}  // namespace
-
 }  // namespace fdi
 
 using namespace fdi;
-
-struct fd_ctx {
-    int device = 0;
-    fd_volume vol{};
-    float* density = nullptr;
-    fd_spectrum spectrum{};
-    fd_kernel kernel{};
-    SpectrumDev spec_dev{};
-    ConeTable cones{};
-    ConeTable* cones_dev = nullptr;
-    int n_fwd = 0;
-    int options = 0;
-
-    // per-call staging of beamlet descriptors (pinned) and scratch
-    BeamletDev* stage = nullptr;
-    size_t stage_cap = 0;
-    hipEvent_t stage_done = nullptr;
-    DevBuf<BeamletDev> desc;
-    DevBuf<float> partial;
-    DevBuf<float> fluence, terma, dens_bev, dose_bev;
-    DevBuf<uint32_t> cand_count, nnz_count;
-    DevBuf<uint64_t> cand_off, nnz_off;
-    DevBuf<uint64_t> tmp_cols;
-    DevBuf<float> tmp_vals;
-    DevBuf<uint64_t> rowoff;          // per-beamlet CSR row offsets of the last chunk
-    uint64_t* host_totals = nullptr;  // pinned, 2 entries
-    // kernel timers: slot k uses ev[2k] / ev[2k+1]
-    hipEvent_t ev[2 * FD_T_COUNT] = {};
-    bool ev_used[FD_T_COUNT] = {};
-    bool timing() const { return (options & FD_OPT_TIMING) != 0 && ev[0] != nullptr; }
-    void mark(int slot, int which, hipStream_t s) {
-        if (!timing()) return;
-        (void)hipEventRecord(ev[2 * slot + which], s);
-        if (which == 1) ev_used[slot] = true;
-    }
-};
-
-struct fd_dosemat {
-    uint64_t n_cols = 0;
-    std::vector<uint64_t> rows;       // numRowsPerMat
-    std::vector<uint64_t> nnz;        // NonZeroElements
-    std::vector<uint64_t> nnz_off;    // start of each matrix in columns/values
-    std::vector<uint64_t> off_off;    // OffsetBufferIdx
-    DevBuf<uint64_t> offsets;         // offsetsBuffer
-    DevBuf<uint64_t> columns;         // columnsBuffer
-    DevBuf<float> values;             // valuesBuffer
-    uint64_t used_off = 0, used_nnz = 0;
-};
 
 namespace {
 
@@ -161,156 +46,12 @@ int check_ctx(fd_ctx* ctx) {
     return FD_OK;
 }
 
-int check_beams(const fd_beam* beams, int n) {
-    if (n < 0 || (n > 0 && !beams)) {
-        set_error("invalid beamlet array");
-        return FD_ERR_ARG;
-    }
-    for (int i = 0; i < n; i++) {
-        if (beams[i].fmap_size[0] != (uint32_t)kFmap || beams[i].fmap_size[1] != (uint32_t)kFmap) {
-            set_error("the HIP kernels implement 16 x 16 BEV grids (subFluenceDim 16)");
-            return FD_ERR_UNSUPPORTED;
-        }
-        if (beams[i].long_dim == 0 || beams[i].long_dim > (1u << 20)) {
-            set_error("beamlet long_dim out of range (run fd_calc_range first)");
-            return FD_ERR_ARG;
-        }
-        if (!(beams[i].sad > 0.0f) || !(beams[i].long_spacing > 0.0f) || !(beams[i].beamlet_size[0] > 0.0f) ||
-            !(beams[i].beamlet_size[1] > 0.0f)) {
-            set_error("beamlet sad, long_spacing and beamlet_size must be positive");
-            return FD_ERR_ARG;
-        }
-    }
-    return FD_OK;
-}
-
 int check_stride(const fd_beam* beams, int n, uint64_t stride) {
     for (int i = 0; i < n; i++)
         if ((uint64_t)beams[i].long_dim * kNpix > stride) {
             set_error("stride smaller than long_dim * 256 of a beamlet");
             return FD_ERR_ARG;
         }
-    return FD_OK;
-}
-
-// Host descriptors -> device, through pinned staging.
-int upload_beams(fd_ctx* ctx, const fd_beam* beams, int n, uint64_t stride, hipStream_t stream) {
-    if (n == 0) return FD_OK;
-    if ((size_t)n > ctx->stage_cap) {
-        if (ctx->stage) {
-            FD_HIP(hipEventSynchronize(ctx->stage_done));
-            FD_HIP(hipHostFree(ctx->stage));
-            ctx->stage = nullptr;
-            ctx->stage_cap = 0;
-        }
-        FD_HIP(hipHostMalloc((void**)&ctx->stage, (size_t)n * sizeof(BeamletDev), hipHostMallocDefault));
-        ctx->stage_cap = n;
-    } else {
-        FD_HIP(hipEventSynchronize(ctx->stage_done));  // previous upload has left the staging area
-    }
-    for (int i = 0; i < n; i++) fill_beamlet_dev(beams[i], (uint64_t)i * stride, ctx->stage[i]);
-    FD_TRY(ctx->desc.reserve(n));
-    FD_HIP(hipMemcpyAsync(ctx->desc.ptr, ctx->stage, (size_t)n * sizeof(BeamletDev), hipMemcpyHostToDevice, stream));
-    FD_HIP(hipEventRecord(ctx->stage_done, stream));
-    return FD_OK;
-}
-
-int max_long_dim(const fd_beam* beams, int n) {
-    int m = 0;
-    for (int i = 0; i < n; i++) m = std::max(m, (int)beams[i].long_dim);
-    return m;
-}
-
-int run_terma(fd_ctx* ctx, int n, const float* fluence, float* terma, float* dens, hipStream_t stream) {
-    ctx->mark(FD_T_TERMA, 0, stream);
-    hipError_t e = launch_terma(ctx->desc.ptr, n, fluence, ctx->density, ctx->vol, ctx->spec_dev,
-                                (ctx->options & FD_OPT_RPL_FP64) != 0, terma, dens, stream);
-    ctx->mark(FD_T_TERMA, 1, stream);
-    if (e != hipSuccess) {
-        set_error(std::string("terma launch: ") + hipGetErrorString(e));
-        return FD_ERR_HIP;
-    }
-    return FD_OK;
-}
-
-int run_cccs(fd_ctx* ctx, int n, int max_L, const float* terma, const float* dens, float* dose, uint64_t stride,
-             hipStream_t stream) {
-    FD_TRY(ctx->partial.reserve((size_t)n * std::max(ctx->n_fwd, 1) * stride));
-    hipEvent_t evs[3];
-    const bool t = ctx->timing();
-    if (t) {
-        evs[0] = ctx->ev[2 * FD_T_CCCS];
-        evs[1] = ctx->ev[2 * FD_T_CCCS + 1];
-        evs[2] = ctx->ev[2 * FD_T_CONE_SUM + 1];
-    }
-    hipError_t e = launch_cccs(ctx->desc.ptr, n, max_L, ctx->cones_dev, ctx->n_fwd, ctx->cones.n_phi,
-                               (ctx->options & FD_OPT_EXACT_G) != 0, terma, dens, ctx->partial.ptr, dose, stride,
-                               stream, t ? evs : nullptr);
-    if (t) {
-        // the cone sum starts where the CCCS kernel stops (fd_ctx_timings pairs them)
-        ctx->ev_used[FD_T_CCCS] = true;
-        ctx->ev_used[FD_T_CONE_SUM] = true;
-    }
-    if (e != hipSuccess) {
-        set_error(std::string("cccs launch: ") + hipGetErrorString(e));
-        return FD_ERR_HIP;
-    }
-    return FD_OK;
-}
-
-// Stages 1-2 of the resampler; leaves the candidate/nonzero offsets in ctx and returns the
-// nonzero total (synchronises the stream twice to size the buffers).
-int run_resample_values(fd_ctx* ctx, int n, const float* dose, float extent, uint64_t* nnz_total,
-                        hipStream_t stream) {
-    const size_t Z = ctx->vol.dim[2];
-    const size_t segs = (size_t)n * Z;
-    FD_TRY(ctx->cand_count.reserve(segs));
-    FD_TRY(ctx->nnz_count.reserve(segs));
-    FD_TRY(ctx->cand_off.reserve(segs + 1));
-    FD_TRY(ctx->nnz_off.reserve(segs + 1));
-    ctx->mark(FD_T_RESAMPLE_COUNT, 0, stream);
-    hipError_t e = launch_resample_count(ctx->desc.ptr, n, ctx->vol, ctx->cand_count.ptr, stream);
-    if (e == hipSuccess) e = launch_scan_u32(ctx->cand_count.ptr, segs, ctx->cand_off.ptr, ctx->cand_off.ptr + segs,
-                                             stream);
-    ctx->mark(FD_T_RESAMPLE_COUNT, 1, stream);
-    if (e != hipSuccess) {
-        set_error(std::string("resample count: ") + hipGetErrorString(e));
-        return FD_ERR_HIP;
-    }
-    FD_HIP(hipMemcpyAsync(ctx->host_totals, ctx->cand_off.ptr + segs, sizeof(uint64_t), hipMemcpyDeviceToHost,
-                          stream));
-    FD_HIP(hipStreamSynchronize(stream));
-    const uint64_t ncand = ctx->host_totals[0];
-    FD_TRY(ctx->tmp_cols.reserve(std::max<uint64_t>(ncand, 1)));
-    FD_TRY(ctx->tmp_vals.reserve(std::max<uint64_t>(ncand, 1)));
-    ctx->mark(FD_T_RESAMPLE_VALUES, 0, stream);
-    e = launch_resample_values(ctx->desc.ptr, n, ctx->vol, extent, dose, ctx->cand_off.ptr, ctx->tmp_cols.ptr,
-                               ctx->tmp_vals.ptr, ctx->nnz_count.ptr, stream);
-    if (e == hipSuccess) e = launch_scan_u32(ctx->nnz_count.ptr, segs, ctx->nnz_off.ptr, ctx->nnz_off.ptr + segs,
-                                             stream);
-    ctx->mark(FD_T_RESAMPLE_VALUES, 1, stream);
-    if (e != hipSuccess) {
-        set_error(std::string("resample values: ") + hipGetErrorString(e));
-        return FD_ERR_HIP;
-    }
-    FD_HIP(hipMemcpyAsync(ctx->host_totals + 1, ctx->nnz_off.ptr + segs, sizeof(uint64_t), hipMemcpyDeviceToHost,
-                          stream));
-    FD_HIP(hipStreamSynchronize(stream));
-    *nnz_total = ctx->host_totals[1];
-    return FD_OK;
-}
-
-int run_resample_compact(fd_ctx* ctx, int n, uint64_t* offsets, uint64_t* columns, float* values,
-                         hipStream_t stream) {
-    ctx->mark(FD_T_RESAMPLE_COMPACT, 0, stream);
-    hipError_t e = launch_resample_compact(n, (int)ctx->vol.dim[2], ctx->cand_off.ptr, ctx->nnz_off.ptr,
-                                           ctx->nnz_count.ptr, ctx->tmp_cols.ptr, ctx->tmp_vals.ptr, offsets,
-                                           columns, values, stream);
-    ctx->mark(FD_T_RESAMPLE_COMPACT, 1, stream);
-    if (e != hipSuccess) {
-        set_error(std::string("resample compact: ") + hipGetErrorString(e));
-        return FD_ERR_HIP;
-    }
     return FD_OK;
 }
 
@@ -341,23 +82,6 @@ ConeTable build_cones(const fd_kernel& k, int* n_fwd) {
     }
     *n_fwd = c.n_fwd;
     return c;
-}
-
-int append_matrix(fd_dosemat* m, uint64_t rows, uint64_t nnz, hipStream_t stream, uint64_t** off_dst,
-                  uint64_t** col_dst, float** val_dst) {
-    FD_TRY(grow_keep(m->offsets, m->used_off, m->used_off + rows + 1, stream));
-    FD_TRY(grow_keep(m->columns, m->used_nnz, m->used_nnz + std::max<uint64_t>(nnz, 1), stream));
-    FD_TRY(grow_keep(m->values, m->used_nnz, m->used_nnz + std::max<uint64_t>(nnz, 1), stream));
-    *off_dst = m->offsets.ptr + m->used_off;
-    *col_dst = m->columns.ptr + m->used_nnz;
-    *val_dst = m->values.ptr + m->used_nnz;
-    m->rows.push_back(rows);
-    m->nnz.push_back(nnz);
-    m->off_off.push_back(m->used_off);
-    m->nnz_off.push_back(m->used_nnz);
-    m->used_off += rows + 1;
-    m->used_nnz += nnz;
-    return FD_OK;
 }
 
 bool write_file(const std::string& path, const void* data, size_t bytes) {
@@ -443,6 +167,47 @@ int fd_beamlet_flags_host(const fd_bundle* bundle, const uint8_t* ptv_mask, cons
         return FD_ERR_ARG;
     }
     return beamlet_flags(*bundle, ptv_mask, *vol, flags);
+}
+
+int fd_beamlet_flags(fd_ctx* ctx, const uint8_t* mask_dev, const fd_bundle* bundles, int n_bundles,
+                     uint8_t* flags_host, void* stream) {
+    FD_TRY(check_ctx(ctx));
+    if (n_bundles < 0 || n_bundles > 65535 || (n_bundles > 0 && (!mask_dev || !bundles || !flags_host))) {
+        set_error("null argument or n_bundles outside [0, 65535]");
+        return FD_ERR_ARG;
+    }
+    if (n_bundles == 0) return FD_OK;
+    std::vector<FlagBeamDev> d((size_t)n_bundles);
+    uint64_t total = 0;
+    int max_pixels = 0;
+    for (int b = 0; b < n_bundles; b++) {
+        const int fx = bundles[b].fluence_dim[0], fy = bundles[b].fluence_dim[1];
+        if (fx <= 0 || fy <= 0 || fx > 1024 || fy > 1024) {
+            set_error("fluence_dim must be in [1, 1024]");
+            return FD_ERR_ARG;
+        }
+        (void)flag_beam_setup(bundles[b], ctx->vol, d[b]);  // a miss leaves L = 0: all flags 0
+        d[b].out_off = (uint32_t)total;
+        total += (uint64_t)fx * fy;
+        max_pixels = std::max(max_pixels, fx * fy);
+    }
+    if (total >= (1ull << 32)) {
+        set_error("too many flags");
+        return FD_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    FD_TRY(ctx->flag_desc.reserve(d.size()));
+    FD_TRY(ctx->flag_out.reserve(total));
+    FD_HIP(hipMemcpyAsync(ctx->flag_desc.ptr, d.data(), d.size() * sizeof(FlagBeamDev), hipMemcpyHostToDevice, s));
+    hipError_t e = launch_beamlet_flags(ctx->flag_desc.ptr, n_bundles, max_pixels, mask_dev, ctx->vol,
+                                        ctx->flag_out.ptr, s);
+    if (e != hipSuccess) {
+        set_error(std::string("beamlet flags launch: ") + hipGetErrorString(e));
+        return FD_ERR_HIP;
+    }
+    FD_HIP(hipMemcpyAsync(flags_host, ctx->flag_out.ptr, total, hipMemcpyDeviceToHost, s));
+    FD_HIP(hipStreamSynchronize(s));
+    return FD_OK;
 }
 
 int fd_read_spectrum(const char* path, fd_spectrum* out) {
@@ -582,12 +347,13 @@ int fd_ctx_create(int device, const fd_volume* vol, const float* density_host, c
     if ((e = hipMalloc((void**)&c->cones_dev, sizeof(ConeTable))) != hipSuccess) return fail(e, "hipMalloc cones");
     if ((e = hipMemcpy(c->cones_dev, &c->cones, sizeof(ConeTable), hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "cone upload");
-    if ((e = hipEventCreateWithFlags(&c->stage_done, hipEventDisableTiming)) != hipSuccess)
-        return fail(e, "hipEventCreate");
-    if ((e = hipHostMalloc((void**)&c->host_totals, 4 * sizeof(uint64_t), hipHostMallocDefault)) != hipSuccess)
-        return fail(e, "hipHostMalloc");
-    for (auto& ev : c->ev)
-        if ((e = hipEventCreate(&ev)) != hipSuccess) return fail(e, "hipEventCreate");
+    {
+        const int r = c->ws.init();
+        if (r != FD_OK) {
+            fd_ctx_destroy(c);
+            return r;
+        }
+    }
     *out = c;
     return FD_OK;
 }
@@ -598,24 +364,9 @@ int fd_ctx_destroy(fd_ctx* ctx) {
     (void)hipDeviceSynchronize();
     if (ctx->density) (void)hipFree(ctx->density);
     if (ctx->cones_dev) (void)hipFree(ctx->cones_dev);
-    if (ctx->stage) (void)hipHostFree(ctx->stage);
-    if (ctx->host_totals) (void)hipHostFree(ctx->host_totals);
-    if (ctx->stage_done) (void)hipEventDestroy(ctx->stage_done);
-    for (auto& ev : ctx->ev)
-        if (ev) (void)hipEventDestroy(ev);
-    ctx->desc.release();
-    ctx->partial.release();
-    ctx->fluence.release();
-    ctx->terma.release();
-    ctx->dens_bev.release();
-    ctx->dose_bev.release();
-    ctx->cand_count.release();
-    ctx->nnz_count.release();
-    ctx->cand_off.release();
-    ctx->nnz_off.release();
-    ctx->tmp_cols.release();
-    ctx->tmp_vals.release();
-    ctx->rowoff.release();
+    ctx->ws.release();
+    ctx->flag_desc.release();
+    ctx->flag_out.release();
     delete ctx;
     return FD_OK;
 }
@@ -625,11 +376,25 @@ int fd_ctx_set_options(fd_ctx* ctx, int options) {
         set_error("null context");
         return FD_ERR_ARG;
     }
-    if (options & ~(FD_OPT_EXACT_G | FD_OPT_RPL_FP64 | FD_OPT_TIMING)) {
+    if (options & ~(FD_OPT_EXACT_G | FD_OPT_RPL_FP64 | FD_OPT_TIMING | FD_OPT_RPL_FP64_ALL)) {
         set_error("unknown option bits");
         return FD_ERR_ARG;
     }
     ctx->options = options;
+    ctx->ws.timing = (options & FD_OPT_TIMING) != 0;
+    return FD_OK;
+}
+
+int fd_ctx_set_threshold(fd_ctx* ctx, float rel) {
+    if (!ctx) {
+        set_error("null context");
+        return FD_ERR_ARG;
+    }
+    if (!(rel >= 0.0f && rel < 1.0f)) {
+        set_error("relative threshold must be in [0, 1)");
+        return FD_ERR_ARG;
+    }
+    ctx->threshold = rel;
     return FD_OK;
 }
 
@@ -640,9 +405,10 @@ int fd_ctx_timings(fd_ctx* ctx, float* ms, int n) {
     }
     for (int k = 0; k < n; k++) {
         ms[k] = -1.0f;
-        if (k >= FD_T_COUNT || !ctx->ev_used[k] || !ctx->ev[0]) continue;
-        hipEvent_t a = (k == FD_T_CONE_SUM) ? ctx->ev[2 * FD_T_CCCS + 1] : ctx->ev[2 * k];
-        hipEvent_t b = ctx->ev[2 * k + 1];
+        const Workspace& w = ctx->ws;
+        if (k >= FD_T_COUNT || !w.ev_used[k] || !w.ev[0]) continue;
+        hipEvent_t a = (k == FD_T_CONE_SUM) ? w.ev[2 * FD_T_CCCS + 1] : w.ev[2 * k];
+        hipEvent_t b = w.ev[2 * k + 1];
         FD_HIP(hipEventSynchronize(b));
         float t = 0.0f;
         FD_HIP(hipEventElapsedTime(&t, a, b));
@@ -671,8 +437,8 @@ int fd_terma(fd_ctx* ctx, const fd_beam* beams, int n, const float* fluence_dev,
         return FD_ERR_ARG;
     }
     hipStream_t s = (hipStream_t)stream;
-    FD_TRY(upload_beams(ctx, beams, n, stride, s));
-    return run_terma(ctx, n, fluence_dev, terma_dev, density_bev_dev, s);
+    FD_TRY(upload_beams(ctx->ws, beams, n, stride, s));
+    return run_terma(ctx, ctx->ws, ctx->options, n, fluence_dev, terma_dev, density_bev_dev, s);
 }
 
 int fd_cccs(fd_ctx* ctx, const fd_beam* beams, int n, const float* terma_dev, const float* density_bev_dev,
@@ -686,8 +452,9 @@ int fd_cccs(fd_ctx* ctx, const fd_beam* beams, int n, const float* terma_dev, co
         return FD_ERR_ARG;
     }
     hipStream_t s = (hipStream_t)stream;
-    FD_TRY(upload_beams(ctx, beams, n, stride, s));
-    return run_cccs(ctx, n, max_long_dim(beams, n), terma_dev, density_bev_dev, dose_bev_dev, stride, s);
+    FD_TRY(upload_beams(ctx->ws, beams, n, stride, s));
+    return run_cccs(ctx, ctx->ws, ctx->options, n, max_long_dim(beams, n), terma_dev, density_bev_dev, dose_bev_dev,
+                    stride, s);
 }
 
 int fd_resample_csr(fd_ctx* ctx, const fd_beam* beams, int n, const float* dose_bev_dev, uint64_t stride,
@@ -707,18 +474,18 @@ int fd_resample_csr(fd_ctx* ctx, const fd_beam* beams, int n, const float* dose_
     out->rows = (uint64_t)n;
     if (n == 0) {
         out->nnz = 0;
-        FD_HIP(hipMemsetAsync(out->offsets, 0, sizeof(uint64_t), s));
+        FD_HIP(launch_fill_u64(out->offsets, 1, 0, s));
         return FD_OK;
     }
-    FD_TRY(upload_beams(ctx, beams, n, stride, s));
+    FD_TRY(upload_beams(ctx->ws, beams, n, stride, s));
     uint64_t nnz = 0;
-    FD_TRY(run_resample_values(ctx, n, dose_bev_dev, extent, &nnz, s));
+    FD_TRY(run_resample_values(ctx, ctx->ws, n, dose_bev_dev, extent, ctx->threshold, &nnz, s));
     out->nnz = nnz;
     if (nnz > out->capacity || (nnz > 0 && (!out->columns || !out->values))) {
         set_error("CSR capacity too small; nnz reported in out->nnz");
         return FD_ERR_CAPACITY;
     }
-    return run_resample_compact(ctx, n, out->offsets, out->columns, out->values, s);
+    return run_resample_compact(ctx, ctx->ws, n, ctx->threshold, out->offsets, out->columns, out->values, s);
 }
 
 int fd_dense_to_csr(const float* dense_dev, uint64_t rows, uint64_t cols, fd_csr* out, void* stream) {
@@ -754,7 +521,7 @@ int fd_dense_to_csr(const float* dense_dev, uint64_t rows, uint64_t cols, fd_csr
     if (rows > 0 && cols > 0) {
         if ((e = launch_dense_count(dense_dev, rows, cols, cnt, s)) != hipSuccess) return hip_fail(e, "count");
     } else if (rows > 0) {
-        if ((e = hipMemsetAsync(cnt, 0, rows * sizeof(uint32_t), s)) != hipSuccess) return hip_fail(e, "memset");
+        if ((e = launch_fill_u32(cnt, rows, 0, s)) != hipSuccess) return hip_fail(e, "fill");
     }
     if ((e = launch_scan_u32(cnt, rows, off, off + rows, s)) != hipSuccess) return hip_fail(e, "scan");
     if ((e = hipMemcpyAsync(host, off + rows, sizeof(uint64_t), hipMemcpyDeviceToHost, s)) != hipSuccess)
@@ -854,6 +621,8 @@ int fd_dosemat_destroy(fd_dosemat* m) {
     m->offsets.release();
     m->columns.release();
     m->values.release();
+    m->values16.release();
+    m->row_scale.release();
     delete m;
     return FD_OK;
 }
@@ -890,61 +659,44 @@ int fd_dosemat_add_beams(fd_ctx* ctx, fd_dosemat* m, const fd_beam* beamlets, co
     }
     if (max_chunk <= 0) max_chunk = 8192;
     hipStream_t s = (hipStream_t)stream;
-    std::vector<uint64_t> host_off, rowoff;
+    std::vector<uint64_t> host_off;
+    BatchResult res;
     int k0 = 0;
     while (k0 < n_beams) {
         // whole beams up to max_chunk beamlets (a larger single beam goes alone)
         int k1 = k0;
         uint64_t n = 0;
         while (k1 < n_beams && (k1 == k0 || n + (uint64_t)counts[k1] <= (uint64_t)max_chunk)) n += counts[k1++];
-        const fd_beam* bl = beamlets + first[k0];
-        uint64_t chunk_nnz = 0;
-        if (n > 0) {
-            const int ni = (int)n;
-            const int maxL = max_long_dim(bl, ni);
-            const uint64_t stride = (uint64_t)maxL * kNpix;
-            FD_TRY(ctx->fluence.reserve((size_t)n * kNpix));
-            FD_TRY(ctx->terma.reserve((size_t)n * stride));
-            FD_TRY(ctx->dens_bev.reserve((size_t)n * stride));
-            FD_TRY(ctx->dose_bev.reserve((size_t)n * stride));
-            FD_TRY(ctx->rowoff.reserve((size_t)n + 1));
-            FD_TRY(upload_beams(ctx, bl, ni, stride, s));
-            FD_HIP(hipMemcpyAsync(ctx->fluence.ptr, fluence + first[k0] * kNpix, (size_t)n * kNpix * sizeof(float),
-                                  hipMemcpyHostToDevice, s));
-            FD_TRY(run_terma(ctx, ni, ctx->fluence.ptr, ctx->terma.ptr, ctx->dens_bev.ptr, s));
-            FD_TRY(run_cccs(ctx, ni, maxL, ctx->terma.ptr, ctx->dens_bev.ptr, ctx->dose_bev.ptr, stride, s));
-            FD_TRY(run_resample_values(ctx, ni, ctx->dose_bev.ptr, extent, &chunk_nnz, s));
-            FD_TRY(grow_keep(m->columns, m->used_nnz, m->used_nnz + std::max<uint64_t>(chunk_nnz, 1), s));
-            FD_TRY(grow_keep(m->values, m->used_nnz, m->used_nnz + std::max<uint64_t>(chunk_nnz, 1), s));
-            FD_TRY(run_resample_compact(ctx, ni, ctx->rowoff.ptr, m->columns.ptr + m->used_nnz,
-                                        m->values.ptr + m->used_nnz, s));
-            rowoff.resize(n + 1);
-            FD_HIP(hipMemcpyAsync(rowoff.data(), ctx->rowoff.ptr, (n + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                                  s));
-            FD_HIP(hipStreamSynchronize(s));
-        } else {
-            rowoff.assign(1, 0);
-        }
+        FD_TRY(run_batch(ctx, ctx->ws, ctx->options, beamlets + first[k0], (int)n, fluence + first[k0] * kNpix,
+                         extent, ctx->threshold, s, &res));
         // one matrix per beam: offsets re-based to each beam's first row
-        const int nb = k1 - k0;
         host_off.clear();
-        host_off.reserve(n + nb);
+        host_off.reserve(n + (k1 - k0));
+        FD_TRY(grow_keep(m->columns, m->used_nnz, m->used_nnz + std::max<uint64_t>(res.nnz, 1), s));
+        FD_TRY(grow_keep(m->values, m->used_nnz, m->used_nnz + std::max<uint64_t>(res.nnz, 1), s));
+        if (res.nnz) {
+            FD_HIP(hipMemcpyAsync(m->columns.ptr + m->used_nnz, ctx->ws.out_cols.ptr, res.nnz * sizeof(uint64_t),
+                                  hipMemcpyDeviceToDevice, s));
+            FD_HIP(hipMemcpyAsync(m->values.ptr + m->used_nnz, ctx->ws.out_vals.ptr, res.nnz * sizeof(float),
+                                  hipMemcpyDeviceToDevice, s));
+        }
         const uint64_t off_base = m->used_off;
         for (int k = k0; k < k1; k++) {
             const uint64_t a = first[k] - first[k0], c = (uint64_t)counts[k];
-            const uint64_t base = rowoff[a];
+            const uint64_t base = res.rowoff[a];
             m->rows.push_back(c);
-            m->nnz.push_back(rowoff[a + c] - base);
+            m->nnz.push_back(res.rowoff[a + c] - base);
             m->off_off.push_back(off_base + host_off.size());
             m->nnz_off.push_back(m->used_nnz + base);
-            for (uint64_t j = 0; j <= c; j++) host_off.push_back(rowoff[a + j] - base);
+            for (uint64_t j = 0; j <= c; j++) host_off.push_back(res.rowoff[a + j] - base);
         }
+        m->packed = -1;  // a packed copy no longer covers the ensemble
         FD_TRY(grow_keep(m->offsets, m->used_off, m->used_off + host_off.size(), s));
         FD_HIP(hipMemcpyAsync(m->offsets.ptr + m->used_off, host_off.data(), host_off.size() * sizeof(uint64_t),
                               hipMemcpyHostToDevice, s));
         FD_HIP(hipStreamSynchronize(s));
         m->used_off += host_off.size();
-        m->used_nnz += chunk_nnz;
+        m->used_nnz += res.nnz;
         k0 = k1;
     }
     return FD_OK;
@@ -1057,6 +809,20 @@ int fd_dosemat_tofile(const fd_dosemat* m, const char* folder) {
         set_error("cannot write the CSR buffers under " + dir);
         return FD_ERR_IO;
     }
+    if (m->packed == FD_VAL_BF16 || m->packed == FD_VAL_FP16) {
+        const uint64_t R = m->total_rows();
+        std::vector<uint16_t> q(m->used_nnz);
+        std::vector<float> sc(R);
+        if (m->used_nnz)
+            FD_HIP(hipMemcpy(q.data(), m->values16.ptr, m->used_nnz * sizeof(uint16_t), hipMemcpyDeviceToHost));
+        if (R) FD_HIP(hipMemcpy(sc.data(), m->row_scale.ptr, R * sizeof(float), hipMemcpyDeviceToHost));
+        const std::string vname = m->packed == FD_VAL_BF16 ? "/valuesBuffer.bf16.bin" : "/valuesBuffer.fp16.bin";
+        if (!write_file(dir + vname, q.data(), q.size() * sizeof(uint16_t)) ||
+            !write_file(dir + "/beamletScale.bin", sc.data(), sc.size() * sizeof(float))) {
+            set_error("cannot write the packed values under " + dir);
+            return FD_ERR_IO;
+        }
+    }
     return FD_OK;
 }
 
@@ -1089,6 +855,7 @@ int fd_dosemat_fromfile(fd_dosemat* m, const char* folder) {
     }
     m->used_off = off_b.size() / 8;
     m->used_nnz = col_b.size() / 8;
+    m->packed = -1;
     if (acc != m->used_nnz) {
         set_error("NonZeroElements does not add up to the columns buffer");
         return FD_ERR_IO;

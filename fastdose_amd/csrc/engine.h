@@ -1,0 +1,199 @@
+// engine.h -- host-side engine of libfastdose_hip: device buffers, per-stream workspaces, the
+// resident context and the stage runners shared by the C-ABI entry points and the serving loop.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "fd_internal.h"
+#include "kernels.h"
+
+namespace fdi {
+
+#define FD_HIP(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            ::fdi::set_error(std::string(#call) + ": " + hipGetErrorString(e_));            \
+            return FD_ERR_HIP;                                                              \
+        }                                                                                   \
+    } while (0)
+
+#define FD_TRY(call)                \
+    do {                            \
+        int r_ = (call);            \
+        if (r_ != FD_OK) return r_; \
+    } while (0)
+
+// Growable device allocation (contents are not kept when it grows).
+template <typename T>
+struct DevBuf {
+    T* ptr = nullptr;
+    size_t cap = 0;  // elements
+    int reserve(size_t n) {
+        if (n <= cap) return FD_OK;
+        size_t want = std::max(n, cap + cap / 2);
+        if (ptr) {
+            hipError_t e = hipFree(ptr);
+            ptr = nullptr;
+            cap = 0;
+            if (e != hipSuccess) {
+                set_error(std::string("hipFree: ") + hipGetErrorString(e));
+                return FD_ERR_HIP;
+            }
+        }
+        hipError_t e = hipMalloc((void**)&ptr, want * sizeof(T));
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+            return FD_ERR_HIP;
+        }
+        cap = want;
+        return FD_OK;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+    size_t bytes() const { return cap * sizeof(T); }
+};
+
+// Growable device buffer that keeps its first `used` elements (used by the dose-matrix ensemble;
+// doubles like MatCSREnsemble::addMat, IMRTSparseMatEns.cu:125-149).
+template <typename T>
+int grow_keep(DevBuf<T>& b, size_t used, size_t need, hipStream_t stream) {
+    if (need <= b.cap) return FD_OK;
+    size_t want = std::max<size_t>(need, b.cap ? b.cap * 2 : 1024);
+    T* np = nullptr;
+    FD_HIP(hipMalloc((void**)&np, want * sizeof(T)));
+    if (used && b.ptr) FD_HIP(hipMemcpyAsync(np, b.ptr, used * sizeof(T), hipMemcpyDeviceToDevice, stream));
+    FD_HIP(hipStreamSynchronize(stream));
+    if (b.ptr) FD_HIP(hipFree(b.ptr));
+    b.ptr = np;
+    b.cap = want;
+    return FD_OK;
+}
+
+// Scratch of one in-flight batch: descriptors, BEV volumes, cone partials, resampler staging and
+// the batch's CSR rows.  One per stream; a context owns one, a server one per slot.
+struct Workspace {
+    BeamletDev* stage = nullptr;  // pinned descriptor staging
+    size_t stage_cap = 0;
+    float* flu_stage = nullptr;   // pinned fluence staging (batched path)
+    size_t flu_cap = 0;
+    hipEvent_t stage_done = nullptr;
+    DevBuf<BeamletDev> desc;
+    DevBuf<float> partial;
+    DevBuf<float> fluence, terma, dens_bev, dose_bev;
+    DevBuf<uint32_t> cand_count, nnz_count;
+    DevBuf<uint64_t> cand_off, nnz_off;
+    DevBuf<uint64_t> tmp_cols;
+    DevBuf<float> tmp_vals;
+    DevBuf<uint64_t> rowoff;      // per-beamlet CSR row offsets of the last batch (n + 1)
+    DevBuf<uint64_t> out_cols;    // CSR of the last batch
+    DevBuf<float> out_vals;
+    DevBuf<uint32_t> flags;       // [0]: resampler staging overflow
+    DevBuf<uint32_t> rowmax;      // per-beamlet maximum (float bits) for the relative threshold
+    uint64_t* host_totals = nullptr;  // pinned: candidates, nonzeros, overflow flag
+    uint64_t* host_rowoff = nullptr;  // pinned copy of rowoff
+    size_t host_rowoff_cap = 0;
+    double cand_per_beamlet = 0.0;    // running estimate that sizes the staging up front
+    // kernel timers: slot k uses ev[2k] / ev[2k+1]
+    hipEvent_t ev[2 * FD_T_COUNT] = {};
+    bool ev_used[FD_T_COUNT] = {};
+    bool timing = false;
+
+    int init();
+    void release();
+    void mark(int slot, int which, hipStream_t s) {
+        if (!timing || !ev[0]) return;
+        (void)hipEventRecord(ev[2 * slot + which], s);
+        if (which == 1) ev_used[slot] = true;
+    }
+    // Device bytes this workspace holds.
+    size_t footprint() const;
+};
+
+// CSR rows of one finished batch, as the host sees them.
+struct BatchResult {
+    std::vector<uint64_t> rowoff;  // n + 1 offsets into ws.out_cols / ws.out_vals
+    uint64_t nnz = 0;
+};
+
+}  // namespace fdi
+
+// Resident state of one device: density, spectrum and cone tables, plus the workspace the
+// synchronous entry points use.
+struct fd_ctx {
+    int device = 0;
+    fd_volume vol{};
+    float* density = nullptr;
+    fd_spectrum spectrum{};
+    fd_kernel kernel{};
+    fdi::SpectrumDev spec_dev{};
+    fdi::ConeTable cones{};
+    fdi::ConeTable* cones_dev = nullptr;
+    int n_fwd = 0;
+    int options = 0;
+    float threshold = 0.0f;  // relative CSR threshold (0: drop exact zeros only)
+    fdi::Workspace ws;
+    fdi::DevBuf<fdi::FlagBeamDev> flag_desc;  // beamlet-flag kernel inputs / outputs
+    fdi::DevBuf<uint8_t> flag_out;
+};
+
+struct fd_dosemat {
+    uint64_t n_cols = 0;
+    std::vector<uint64_t> rows;       // numRowsPerMat
+    std::vector<uint64_t> nnz;        // NonZeroElements
+    std::vector<uint64_t> nnz_off;    // start of each matrix in columns/values
+    std::vector<uint64_t> off_off;    // OffsetBufferIdx
+    fdi::DevBuf<uint64_t> offsets;    // offsetsBuffer
+    fdi::DevBuf<uint64_t> columns;    // columnsBuffer
+    fdi::DevBuf<float> values;        // valuesBuffer
+    uint64_t used_off = 0, used_nnz = 0;
+    int packed = -1;                  // FD_VAL_BF16 / FD_VAL_FP16 once packed, -1 otherwise
+    fdi::DevBuf<uint16_t> values16;   // packed values (entry order of values)
+    fdi::DevBuf<float> row_scale;     // per-row scale of the packed values
+    uint64_t total_rows() const {
+        uint64_t r = 0;
+        for (uint64_t x : rows) r += x;
+        return r;
+    }
+};
+
+namespace fdi {
+
+int check_beams(const fd_beam* beams, int n);
+int max_long_dim(const fd_beam* beams, int n);
+
+// Single stages on a workspace (descriptors uploaded by upload_beams first).
+int upload_beams(Workspace& w, const fd_beam* beams, int n, uint64_t stride, hipStream_t stream);
+int run_terma(const fd_ctx* ctx, Workspace& w, int options, int n, const float* fluence, float* terma, float* dens,
+              hipStream_t stream);
+int run_cccs(const fd_ctx* ctx, Workspace& w, int options, int n, int max_L, const float* terma, const float* dens,
+             float* dose, uint64_t stride, hipStream_t stream);
+// Resampler stages 1-2 with exact sizing (two host synchronisations); leaves the offsets in w.
+// thr > 0 keeps only entries >= thr * (the beamlet's maximum value).
+int run_resample_values(const fd_ctx* ctx, Workspace& w, int n, const float* dose, float extent, float thr,
+                        uint64_t* nnz_total, hipStream_t stream);
+int run_resample_compact(const fd_ctx* ctx, Workspace& w, int n, float thr, uint64_t* offsets, uint64_t* columns,
+                         float* values, hipStream_t stream);
+
+// The whole path for a batch of beamlets (host descriptors and host fluence, 256 floats each):
+// Terma -> CCCS -> resampling, CSR rows into w.out_cols / w.out_vals, one host synchronisation
+// (two when the resampler staging estimate was too small).
+int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams, int n, const float* fluence,
+              float extent, float thr, hipStream_t stream, BatchResult* out);
+
+// Device bytes a batch of these beamlets needs (BEV volumes, cone partials, resampler staging).
+size_t batch_footprint(const fd_ctx* ctx, const fd_beam* beams, int n);
+
+// Appends one matrix of `rows` rows and `nnz` nonzeros; returns where its arrays go.
+int append_matrix(fd_dosemat* m, uint64_t rows, uint64_t nnz, hipStream_t stream, uint64_t** off_dst,
+                  uint64_t** col_dst, float** val_dst);
+
+}  // namespace fdi

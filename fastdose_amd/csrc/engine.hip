@@ -1,0 +1,392 @@
+// engine.hip -- stage runners of libfastdose_hip on a workspace (one stream each).
+//
+// The per-beam pipeline of IMRT::DoseMatConstruction (IMRTOpt/src/IMRTDoseMat.cu:183-268):
+// TermaComputeCollective -> DoseComputeCollective -> BEV2PVCSInterp -> addMat, here for any
+// batch of beamlets (of one or many beams) with one host synchronisation per batch.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "engine.h"
+#include "geometry.h"
+
+namespace fdi {
+
+int Workspace::init() {
+    FD_HIP(hipEventCreateWithFlags(&stage_done, hipEventDisableTiming));
+    FD_HIP(hipHostMalloc((void**)&host_totals, 4 * sizeof(uint64_t), hipHostMallocDefault));
+    for (auto& e : ev) FD_HIP(hipEventCreate(&e));
+    return FD_OK;
+}
+
+void Workspace::release() {
+    if (stage) (void)hipHostFree(stage);
+    if (flu_stage) (void)hipHostFree(flu_stage);
+    if (host_totals) (void)hipHostFree(host_totals);
+    if (host_rowoff) (void)hipHostFree(host_rowoff);
+    host_rowoff = nullptr;
+    host_rowoff_cap = 0;
+    if (stage_done) (void)hipEventDestroy(stage_done);
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    stage = nullptr;
+    flu_stage = nullptr;
+    host_totals = nullptr;
+    stage_done = nullptr;
+    for (auto& e : ev) e = nullptr;
+    desc.release();
+    partial.release();
+    fluence.release();
+    terma.release();
+    dens_bev.release();
+    dose_bev.release();
+    cand_count.release();
+    nnz_count.release();
+    cand_off.release();
+    nnz_off.release();
+    tmp_cols.release();
+    tmp_vals.release();
+    rowoff.release();
+    out_cols.release();
+    out_vals.release();
+    flags.release();
+    rowmax.release();
+}
+
+size_t Workspace::footprint() const {
+    return desc.bytes() + partial.bytes() + fluence.bytes() + terma.bytes() + dens_bev.bytes() + dose_bev.bytes() +
+           cand_count.bytes() + nnz_count.bytes() + cand_off.bytes() + nnz_off.bytes() + tmp_cols.bytes() +
+           tmp_vals.bytes() + rowoff.bytes() + out_cols.bytes() + out_vals.bytes() + flags.bytes() + rowmax.bytes();
+}
+
+int check_beams(const fd_beam* beams, int n) {
+    if (n < 0 || (n > 0 && !beams)) {
+        set_error("invalid beamlet array");
+        return FD_ERR_ARG;
+    }
+    for (int i = 0; i < n; i++) {
+        if (beams[i].fmap_size[0] != (uint32_t)kFmap || beams[i].fmap_size[1] != (uint32_t)kFmap) {
+            set_error("the HIP kernels implement 16 x 16 BEV grids (subFluenceDim 16)");
+            return FD_ERR_UNSUPPORTED;
+        }
+        if (beams[i].long_dim == 0 || beams[i].long_dim > (1u << 20)) {
+            set_error("beamlet long_dim out of range (run fd_calc_range first)");
+            return FD_ERR_ARG;
+        }
+        if (!(beams[i].sad > 0.0f) || !(beams[i].long_spacing > 0.0f) || !(beams[i].beamlet_size[0] > 0.0f) ||
+            !(beams[i].beamlet_size[1] > 0.0f)) {
+            set_error("beamlet sad, long_spacing and beamlet_size must be positive");
+            return FD_ERR_ARG;
+        }
+    }
+    return FD_OK;
+}
+
+int max_long_dim(const fd_beam* beams, int n) {
+    int m = 0;
+    for (int i = 0; i < n; i++) m = std::max(m, (int)beams[i].long_dim);
+    return m;
+}
+
+namespace {
+
+// Pinned staging of the descriptors (and optionally the fluence) of a batch; waits until the
+// previous upload from this workspace has left the staging area.
+int stage_inputs(Workspace& w, const fd_beam* beams, int n, uint64_t stride, const float* fluence_host) {
+    FD_HIP(hipEventSynchronize(w.stage_done));
+    if ((size_t)n > w.stage_cap) {
+        if (w.stage) FD_HIP(hipHostFree(w.stage));
+        w.stage = nullptr;
+        w.stage_cap = 0;
+        FD_HIP(hipHostMalloc((void**)&w.stage, (size_t)n * sizeof(BeamletDev), hipHostMallocDefault));
+        w.stage_cap = n;
+    }
+    for (int i = 0; i < n; i++) fill_beamlet_dev(beams[i], (uint64_t)i * stride, w.stage[i]);
+    if (fluence_host) {
+        const size_t nf = (size_t)n * kNpix;
+        if (nf > w.flu_cap) {
+            if (w.flu_stage) FD_HIP(hipHostFree(w.flu_stage));
+            w.flu_stage = nullptr;
+            w.flu_cap = 0;
+            FD_HIP(hipHostMalloc((void**)&w.flu_stage, nf * sizeof(float), hipHostMallocDefault));
+            w.flu_cap = nf;
+        }
+        std::memcpy(w.flu_stage, fluence_host, nf * sizeof(float));
+    }
+    return FD_OK;
+}
+
+// Voxels a beamlet's widened frustum covers (volume / voxel volume), the scale the resampler's
+// candidate staging is sized from.
+double frustum_voxels(const fd_beam& b, const fd_volume& vol) {
+    const double vx = vol.voxel_size[0], vy = vol.voxel_size[1], vz = vol.voxel_size[2];
+    const double margin = std::sqrt(vx * vx + vy * vy + vz * vz) + 0.02;
+    const double ls = b.long_spacing, sad = b.sad;
+    const double z0 = b.lim_min, z1 = b.lim_min + ls * b.long_dim;
+    // integral of (w(z) + 2 margin)^2 dz with w(z) = 16 bs z / sad (square field)
+    const double k = kFmap * (double)std::max(b.beamlet_size[0], b.beamlet_size[1]) / sad;
+    auto F = [&](double z) {
+        const double u = k * z + 2 * margin;
+        return u * u * u / (3 * k);
+    };
+    return std::max(F(z1) - F(z0), 0.0) / (vx * vy * vz);
+}
+
+}  // namespace
+
+size_t batch_footprint(const fd_ctx* ctx, const fd_beam* beams, int n) {
+    if (n <= 0) return 0;
+    const uint64_t stride = (uint64_t)max_long_dim(beams, n) * kNpix;
+    double cand = 0;
+    for (int i = 0; i < n; i++) cand += frustum_voxels(beams[i], ctx->vol);
+    const size_t segs = (size_t)n * ctx->vol.dim[2];
+    return (size_t)n * stride * sizeof(float) * (3 + std::max(ctx->n_fwd, 1)) + (size_t)n * kNpix * sizeof(float) +
+           (size_t)(2.0 * cand) * (2 * sizeof(uint64_t) + 2 * sizeof(float)) + segs * 24 + (size_t)n * 128;
+}
+
+int upload_beams(Workspace& w, const fd_beam* beams, int n, uint64_t stride, hipStream_t stream) {
+    if (n == 0) return FD_OK;
+    FD_TRY(stage_inputs(w, beams, n, stride, nullptr));
+    FD_TRY(w.desc.reserve(n));
+    FD_HIP(hipMemcpyAsync(w.desc.ptr, w.stage, (size_t)n * sizeof(BeamletDev), hipMemcpyHostToDevice, stream));
+    FD_HIP(hipEventRecord(w.stage_done, stream));
+    return FD_OK;
+}
+
+int run_terma(const fd_ctx* ctx, Workspace& w, int options, int n, const float* fluence, float* terma, float* dens,
+              hipStream_t stream) {
+    // FP64 radiological path on rays longer than kLongRayVoxels voxels (or on all rays)
+    const float vmin = std::min(ctx->vol.voxel_size[0], std::min(ctx->vol.voxel_size[1], ctx->vol.voxel_size[2]));
+    const float rpl64_len = (options & FD_OPT_RPL_FP64_ALL) ? -1.0f
+                            : (options & FD_OPT_RPL_FP64) ? kLongRayVoxels * vmin
+                                                          : INFINITY;
+    w.mark(FD_T_TERMA, 0, stream);
+    hipError_t e = launch_terma(w.desc.ptr, n, fluence, ctx->density, ctx->vol, ctx->spec_dev, rpl64_len, terma,
+                                dens, stream);
+    w.mark(FD_T_TERMA, 1, stream);
+    if (e != hipSuccess) {
+        set_error(std::string("terma launch: ") + hipGetErrorString(e));
+        return FD_ERR_HIP;
+    }
+    return FD_OK;
+}
+
+int run_cccs(const fd_ctx* ctx, Workspace& w, int options, int n, int max_L, const float* terma, const float* dens,
+             float* dose, uint64_t stride, hipStream_t stream) {
+    FD_TRY(w.partial.reserve((size_t)n * std::max(ctx->n_fwd, 1) * stride));
+    hipEvent_t evs[3];
+    const bool t = w.timing && w.ev[0];
+    if (t) {
+        evs[0] = w.ev[2 * FD_T_CCCS];
+        evs[1] = w.ev[2 * FD_T_CCCS + 1];
+        evs[2] = w.ev[2 * FD_T_CONE_SUM + 1];
+    }
+    hipError_t e = launch_cccs(w.desc.ptr, n, max_L, ctx->cones_dev, ctx->n_fwd, ctx->cones.n_phi,
+                               (options & FD_OPT_EXACT_G) != 0, terma, dens, w.partial.ptr, dose, stride, stream,
+                               t ? evs : nullptr);
+    if (t) {
+        // the cone sum starts where the CCCS kernel stops (timings pair them)
+        w.ev_used[FD_T_CCCS] = true;
+        w.ev_used[FD_T_CONE_SUM] = true;
+    }
+    if (e != hipSuccess) {
+        set_error(std::string("cccs launch: ") + hipGetErrorString(e));
+        return FD_ERR_HIP;
+    }
+    return FD_OK;
+}
+
+namespace {
+
+int reserve_segments(const fd_ctx* ctx, Workspace& w, int n) {
+    const size_t segs = (size_t)n * ctx->vol.dim[2];
+    FD_TRY(w.cand_count.reserve(segs));
+    FD_TRY(w.nnz_count.reserve(segs));
+    FD_TRY(w.cand_off.reserve(segs + 1));
+    FD_TRY(w.nnz_off.reserve(segs + 1));
+    FD_TRY(w.flags.reserve(1));
+    FD_TRY(w.rowmax.reserve(n));
+    return FD_OK;
+}
+
+int launch_count(const fd_ctx* ctx, Workspace& w, int n, hipStream_t stream) {
+    const size_t segs = (size_t)n * ctx->vol.dim[2];
+    w.mark(FD_T_RESAMPLE_COUNT, 0, stream);
+    hipError_t e = launch_resample_count(w.desc.ptr, n, ctx->vol, w.cand_count.ptr, stream);
+    if (e == hipSuccess) e = launch_scan_u32(w.cand_count.ptr, segs, w.cand_off.ptr, w.cand_off.ptr + segs, stream);
+    w.mark(FD_T_RESAMPLE_COUNT, 1, stream);
+    if (e != hipSuccess) {
+        set_error(std::string("resample count: ") + hipGetErrorString(e));
+        return FD_ERR_HIP;
+    }
+    return FD_OK;
+}
+
+int launch_values(const fd_ctx* ctx, Workspace& w, int n, const float* dose, float extent, float thr, uint64_t cap,
+                  hipStream_t stream) {
+    const size_t segs = (size_t)n * ctx->vol.dim[2];
+    const int Z = (int)ctx->vol.dim[2];
+    w.mark(FD_T_RESAMPLE_VALUES, 0, stream);
+    hipError_t e = launch_fill_u32(w.flags.ptr, 1, 0u, stream);
+    if (e == hipSuccess && thr > 0.0f) e = launch_fill_u32(w.rowmax.ptr, (uint64_t)n, 0u, stream);
+    if (e == hipSuccess)
+        e = launch_resample_values(w.desc.ptr, n, ctx->vol, extent, dose, w.cand_off.ptr, cap, w.flags.ptr,
+                                   w.tmp_cols.ptr, w.tmp_vals.ptr, w.nnz_count.ptr,
+                                   thr > 0.0f ? w.rowmax.ptr : nullptr, stream);
+    if (thr > 0.0f) {
+        // kept entries per segment into cand_count (free once cand_off is scanned), then offsets
+        if (e == hipSuccess)
+            e = launch_resample_threshold_count(n, Z, w.cand_off.ptr, w.nnz_count.ptr, w.tmp_vals.ptr, w.rowmax.ptr,
+                                                thr, w.cand_count.ptr, stream);
+        if (e == hipSuccess)
+            e = launch_scan_u32(w.cand_count.ptr, segs, w.nnz_off.ptr, w.nnz_off.ptr + segs, stream);
+    } else if (e == hipSuccess) {
+        e = launch_scan_u32(w.nnz_count.ptr, segs, w.nnz_off.ptr, w.nnz_off.ptr + segs, stream);
+    }
+    w.mark(FD_T_RESAMPLE_VALUES, 1, stream);
+    if (e != hipSuccess) {
+        set_error(std::string("resample values: ") + hipGetErrorString(e));
+        return FD_ERR_HIP;
+    }
+    return FD_OK;
+}
+
+// totals of the last resampler pass -> pinned host: [0] candidates, [1] nonzeros, [2] overflow;
+// with rows, also the batch's row offsets -> w.host_rowoff
+int fetch_totals(const fd_ctx* ctx, Workspace& w, int n, hipStream_t stream, bool rows) {
+    const size_t segs = (size_t)n * ctx->vol.dim[2];
+    w.host_totals[2] = 0;
+    if (rows) {
+        if ((size_t)n + 1 > w.host_rowoff_cap) {
+            if (w.host_rowoff) FD_HIP(hipHostFree(w.host_rowoff));
+            w.host_rowoff = nullptr;
+            w.host_rowoff_cap = 0;
+            const size_t want = std::max<size_t>((size_t)n + 1, 2 * w.host_rowoff_cap);
+            FD_HIP(hipHostMalloc((void**)&w.host_rowoff, want * sizeof(uint64_t), hipHostMallocDefault));
+            w.host_rowoff_cap = want;
+        }
+        FD_HIP(hipMemcpyAsync(w.host_rowoff, w.rowoff.ptr, ((size_t)n + 1) * sizeof(uint64_t),
+                              hipMemcpyDeviceToHost, stream));
+    }
+    FD_HIP(hipMemcpyAsync(w.host_totals, w.cand_off.ptr + segs, sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+    FD_HIP(hipMemcpyAsync(w.host_totals + 1, w.nnz_off.ptr + segs, sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+    FD_HIP(hipMemcpyAsync(w.host_totals + 2, w.flags.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    return FD_OK;
+}
+
+}  // namespace
+
+int run_resample_values(const fd_ctx* ctx, Workspace& w, int n, const float* dose, float extent, float thr,
+                        uint64_t* nnz_total, hipStream_t stream) {
+    const size_t segs = (size_t)n * ctx->vol.dim[2];
+    FD_TRY(reserve_segments(ctx, w, n));
+    FD_TRY(launch_count(ctx, w, n, stream));
+    FD_HIP(hipMemcpyAsync(w.host_totals, w.cand_off.ptr + segs, sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+    FD_HIP(hipStreamSynchronize(stream));
+    const uint64_t ncand = w.host_totals[0];
+    FD_TRY(w.tmp_cols.reserve(std::max<uint64_t>(ncand, 1)));
+    FD_TRY(w.tmp_vals.reserve(std::max<uint64_t>(ncand, 1)));
+    FD_TRY(launch_values(ctx, w, n, dose, extent, thr, ncand, stream));
+    FD_HIP(hipMemcpyAsync(w.host_totals + 1, w.nnz_off.ptr + segs, sizeof(uint64_t), hipMemcpyDeviceToHost,
+                          stream));
+    FD_HIP(hipStreamSynchronize(stream));
+    *nnz_total = w.host_totals[1];
+    return FD_OK;
+}
+
+int run_resample_compact(const fd_ctx* ctx, Workspace& w, int n, float thr, uint64_t* offsets, uint64_t* columns,
+                         float* values, hipStream_t stream) {
+    w.mark(FD_T_RESAMPLE_COMPACT, 0, stream);
+    hipError_t e = launch_resample_compact(n, (int)ctx->vol.dim[2], w.cand_off.ptr, w.nnz_off.ptr, w.nnz_count.ptr,
+                                           w.tmp_cols.ptr, w.tmp_vals.ptr, thr > 0.0f ? w.rowmax.ptr : nullptr, thr,
+                                           offsets, columns, values, stream);
+    w.mark(FD_T_RESAMPLE_COMPACT, 1, stream);
+    if (e != hipSuccess) {
+        set_error(std::string("resample compact: ") + hipGetErrorString(e));
+        return FD_ERR_HIP;
+    }
+    return FD_OK;
+}
+
+int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams, int n, const float* fluence,
+              float extent, float thr, hipStream_t stream, BatchResult* out) {
+    out->rowoff.assign(1, 0);
+    out->nnz = 0;
+    if (n == 0) return FD_OK;
+    const int maxL = max_long_dim(beams, n);
+    const uint64_t stride = (uint64_t)maxL * kNpix;
+    double est = 0;
+    for (int i = 0; i < n; i++) est += frustum_voxels(beams[i], ctx->vol);
+    if (w.cand_per_beamlet <= 0) w.cand_per_beamlet = 2.0;  // candidates per frustum voxel, first guess
+    uint64_t cap = std::max<uint64_t>((uint64_t)std::ceil(w.cand_per_beamlet * est), 1024);
+
+    FD_TRY(w.desc.reserve(n));
+    FD_TRY(w.fluence.reserve((size_t)n * kNpix));
+    FD_TRY(w.terma.reserve((size_t)n * stride));
+    FD_TRY(w.dens_bev.reserve((size_t)n * stride));
+    FD_TRY(w.dose_bev.reserve((size_t)n * stride));
+    FD_TRY(w.rowoff.reserve((size_t)n + 1));
+    FD_TRY(reserve_segments(ctx, w, n));
+    FD_TRY(w.tmp_cols.reserve(cap));
+    FD_TRY(w.tmp_vals.reserve(cap));
+    FD_TRY(w.out_cols.reserve(cap));
+    FD_TRY(w.out_vals.reserve(cap));
+
+    FD_TRY(stage_inputs(w, beams, n, stride, fluence));
+    FD_HIP(hipMemcpyAsync(w.desc.ptr, w.stage, (size_t)n * sizeof(BeamletDev), hipMemcpyHostToDevice, stream));
+    FD_HIP(hipMemcpyAsync(w.fluence.ptr, w.flu_stage, (size_t)n * kNpix * sizeof(float), hipMemcpyHostToDevice,
+                          stream));
+    FD_HIP(hipEventRecord(w.stage_done, stream));
+    FD_TRY(run_terma(ctx, w, options, n, w.fluence.ptr, w.terma.ptr, w.dens_bev.ptr, stream));
+    FD_TRY(run_cccs(ctx, w, options, n, maxL, w.terma.ptr, w.dens_bev.ptr, w.dose_bev.ptr, stride, stream));
+    FD_TRY(launch_count(ctx, w, n, stream));
+    FD_TRY(launch_values(ctx, w, n, w.dose_bev.ptr, extent, thr, cap, stream));
+    FD_TRY(run_resample_compact(ctx, w, n, thr, w.rowoff.ptr, w.out_cols.ptr, w.out_vals.ptr, stream));
+    FD_TRY(fetch_totals(ctx, w, n, stream, true));
+    FD_HIP(hipStreamSynchronize(stream));
+    const uint64_t ncand = w.host_totals[0];
+    const double ratio = est > 0 ? (double)ncand / est : 1.0;
+    if (w.host_totals[2]) {
+        // staging estimate too small: exact size now, re-run the values pass
+        cap = ncand;
+        FD_TRY(w.tmp_cols.reserve(cap));
+        FD_TRY(w.tmp_vals.reserve(cap));
+        FD_TRY(w.out_cols.reserve(cap));
+        FD_TRY(w.out_vals.reserve(cap));
+        FD_TRY(launch_values(ctx, w, n, w.dose_bev.ptr, extent, thr, cap, stream));
+        FD_TRY(run_resample_compact(ctx, w, n, thr, w.rowoff.ptr, w.out_cols.ptr, w.out_vals.ptr, stream));
+        FD_TRY(fetch_totals(ctx, w, n, stream, true));
+        FD_HIP(hipStreamSynchronize(stream));
+        if (w.host_totals[2]) {
+            set_error("resampler staging overflow with exact sizing");
+            return FD_ERR_HIP;
+        }
+    }
+    // follow the observed candidate density, with headroom; shrink slowly
+    w.cand_per_beamlet = std::max(1.15 * ratio, 0.95 * w.cand_per_beamlet);
+    if (w.cand_per_beamlet > 1.5 * 1.15 * ratio) w.cand_per_beamlet = 1.5 * 1.15 * ratio;
+    out->nnz = w.host_totals[1];
+    out->rowoff.assign(w.host_rowoff, w.host_rowoff + n + 1);
+    return FD_OK;
+}
+
+int append_matrix(fd_dosemat* m, uint64_t rows, uint64_t nnz, hipStream_t stream, uint64_t** off_dst,
+                  uint64_t** col_dst, float** val_dst) {
+    FD_TRY(grow_keep(m->offsets, m->used_off, m->used_off + rows + 1, stream));
+    FD_TRY(grow_keep(m->columns, m->used_nnz, m->used_nnz + std::max<uint64_t>(nnz, 1), stream));
+    FD_TRY(grow_keep(m->values, m->used_nnz, m->used_nnz + std::max<uint64_t>(nnz, 1), stream));
+    m->packed = -1;  // a packed copy no longer covers the ensemble
+    *off_dst = m->offsets.ptr + m->used_off;
+    *col_dst = m->columns.ptr + m->used_nnz;
+    *val_dst = m->values.ptr + m->used_nnz;
+    m->rows.push_back(rows);
+    m->nnz.push_back(nnz);
+    m->off_off.push_back(m->used_off);
+    m->nnz_off.push_back(m->used_nnz);
+    m->used_off += rows + 1;
+    m->used_nnz += nnz;
+    return FD_OK;
+}
+
+}  // namespace fdi

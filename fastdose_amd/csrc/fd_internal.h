@@ -26,6 +26,7 @@ constexpr int kResampleSS = 5;    // BEV -> PVCS supersampling per axis (IMRTgeo
 constexpr int kMaxStepsPerSlice = 1 << 16;
 constexpr double kMinCosTheta = 1e-2;
 
+constexpr float kLongRayVoxels = 256.0f;  // FD_OPT_RPL_FP64: FP64 path on rays longer than this
 constexpr float kEps = 1e-4f;        // eps_fastdose (macros.h.in:26)
 constexpr float kLargerEps = 1e-3f;  // larger_eps_fastdose (macros.h.in:27)
 
@@ -46,6 +47,20 @@ struct BeamletDev {
     uint64_t bev_off;  // float offset of this beamlet's BEV volume
 };
 static_assert(sizeof(BeamletDev) % 8 == 0, "BeamletDev alignment");
+
+// Per-bundle input of the beamlet-PTV intersection kernel (d_beamletFlagInit,
+// IMRTBeamBundle.cu:130-316): the bundle's central beam after calc_range.
+struct FlagBeamDev {
+    float Ri[9];       // BEV -> PVCS rotation, row-major
+    float src[3];
+    float bs[2];       // beamlet_size of the bundle
+    float sad;
+    float lim_min;
+    float ls;
+    int32_t L;         // long_dim (0: the central beam misses the volume, all flags 0)
+    int32_t fx, fy;    // fluence_dim
+    uint32_t out_off;  // first flag of this bundle in the output
+};
 
 // Spectrum folded for the Terma loop: w_e = fluence_e * E_e * mu_e and mu_e.
 struct SpectrumDev {

@@ -272,7 +272,7 @@ bool mask_hit(const uint8_t* mask, const fd_volume& vol, const float c[3]) {
 
 }  // namespace
 
-int beamlet_flags(const fd_bundle& bb, const uint8_t* mask, const fd_volume& vol, uint8_t* flags) {
+int flag_beam_setup(const fd_bundle& bb, const fd_volume& vol, FlagBeamDev& d) {
     fd_beam beam;
     std::memset(&beam, 0, sizeof(beam));
     for (int i = 0; i < 3; i++) {
@@ -285,31 +285,46 @@ int beamlet_flags(const fd_bundle& bb, const uint8_t* mask, const fd_volume& vol
     beam.fmap_size[1] = (uint32_t)bb.fluence_dim[1];
     beam.sad = bb.sad;
     beam.long_spacing = bb.long_spacing;
+    std::memset(&d, 0, sizeof(d));
+    d.fx = bb.fluence_dim[0];
+    d.fy = bb.fluence_dim[1];
+    const int r = calc_range(beam, vol);
+    beam_matrix(beam.angles[0], beam.angles[1], beam.angles[2], true, d.Ri);
+    for (int i = 0; i < 3; i++) d.src[i] = beam.source[i];
+    d.bs[0] = beam.beamlet_size[0];
+    d.bs[1] = beam.beamlet_size[1];
+    d.sad = beam.sad;
+    d.lim_min = beam.lim_min;
+    d.ls = beam.long_spacing;
+    d.L = r == FD_OK ? (int32_t)beam.long_dim : 0;
+    return r;
+}
+
+int beamlet_flags(const fd_bundle& bb, const uint8_t* mask, const fd_volume& vol, uint8_t* flags) {
+    FlagBeamDev d;
     const int fx = bb.fluence_dim[0], fy = bb.fluence_dim[1];
-    if (calc_range(beam, vol) != FD_OK) {
+    if (flag_beam_setup(bb, vol, d) != FD_OK) {
         std::fill(flags, flags + (size_t)fx * fy, (uint8_t)0);
         return FD_ERR_GEOM;
     }
-    float Ri[9];
-    beam_matrix(beam.angles[0], beam.angles[1], beam.angles[2], true, Ri);
     const int ss = 3;  // lateral supersampling of d_beamletFlagInit (IMRTBeamBundle.cu:238)
     for (int idx = 0; idx < fx * fy; idx++) {
         const int bx = idx % fx, by = idx / fx;
         bool on = false;
-        for (uint32_t i = 0; i < beam.long_dim && !on; i++) {
-            const float depth = beam.lim_min + i * beam.long_spacing;
-            const float scale = depth / beam.sad;
+        for (uint32_t i = 0; i < (uint32_t)d.L && !on; i++) {
+            const float depth = d.lim_min + i * d.ls;
+            const float scale = depth / d.sad;
             for (int j = 0; j < ss && !on; j++) {
                 for (int k = 0; k < ss && !on; k++) {
-                    const float dx = (bx + (j + 0.5f) / ss - fx * 0.5f) * beam.beamlet_size[0];
-                    const float dy = beam.sad;
-                    const float dz = (by + (k + 0.5f) / ss - fy * 0.5f) * beam.beamlet_size[1];
-                    const float px = fmaf(Ri[2], dz, fmaf(Ri[1], dy, Ri[0] * dx));
-                    const float py = fmaf(Ri[5], dz, fmaf(Ri[4], dy, Ri[3] * dx));
-                    const float pz = fmaf(Ri[8], dz, fmaf(Ri[7], dy, Ri[6] * dx));
-                    const float cpos[3] = {fmaf(px, scale, beam.source[0]) / vol.voxel_size[0],
-                                           fmaf(py, scale, beam.source[1]) / vol.voxel_size[1],
-                                           fmaf(pz, scale, beam.source[2]) / vol.voxel_size[2]};
+                    const float dx = (bx + (j + 0.5f) / ss - fx * 0.5f) * d.bs[0];
+                    const float dy = d.sad;
+                    const float dz = (by + (k + 0.5f) / ss - fy * 0.5f) * d.bs[1];
+                    const float px = fmaf(d.Ri[2], dz, fmaf(d.Ri[1], dy, d.Ri[0] * dx));
+                    const float py = fmaf(d.Ri[5], dz, fmaf(d.Ri[4], dy, d.Ri[3] * dx));
+                    const float pz = fmaf(d.Ri[8], dz, fmaf(d.Ri[7], dy, d.Ri[6] * dx));
+                    const float cpos[3] = {fmaf(px, scale, d.src[0]) / vol.voxel_size[0],
+                                           fmaf(py, scale, d.src[1]) / vol.voxel_size[1],
+                                           fmaf(pz, scale, d.src[2]) / vol.voxel_size[2]};
                     on = mask_hit(mask, vol, cpos);
                 }
             }

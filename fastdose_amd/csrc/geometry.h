@@ -13,6 +13,8 @@ void beam_matrix(double gantry, double couch, double coll, bool to_pvcs, float o
 int calc_range(fd_beam& beam, const fd_volume& vol);
 int beamlet_init(const fd_bundle& bb, int ix, int iy, const fd_volume& vol, fd_beam& out, float* fluence);
 int beamlet_flags(const fd_bundle& bb, const uint8_t* mask, const fd_volume& vol, uint8_t* flags);
+// Central beam of a bundle for the flag kernel (calc_range status returned; L = 0 on a miss).
+int flag_beam_setup(const fd_bundle& bb, const fd_volume& vol, FlagBeamDev& d);
 int ptv_isocenter(const uint8_t* mask, const fd_volume& vol, float iso[3]);
 
 void fill_beamlet_dev(const fd_beam& b, uint64_t bev_off, BeamletDev& d);

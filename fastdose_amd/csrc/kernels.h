@@ -7,9 +7,15 @@
 
 namespace fdi {
 
+// Beamlet / PTV intersection flags of n bundles (d_beamletFlagInit, IMRTBeamBundle.cu:130-316);
+// flags[beams[b].out_off + pixel], max_pixels = the largest fluence_dim^2.
+hipError_t launch_beamlet_flags(const FlagBeamDev* beams, int n, int max_pixels, const uint8_t* mask,
+                                const fd_volume& vol, uint8_t* flags, hipStream_t stream);
+
 // Terma + mean density along the 256 BEV rays of each beamlet (TermaComp.cu:179-253).
+// Beamlets with long_dim * long_spacing > rpl64_len [cm] accumulate the radiological path in FP64.
 hipError_t launch_terma(const BeamletDev* beams, int n, const float* fluence, const float* density,
-                        const fd_volume& vol, const SpectrumDev& spec, bool rpl_fp64, float* terma,
+                        const fd_volume& vol, const SpectrumDev& spec, float rpl64_len, float* terma,
                         float* dens_bev, hipStream_t stream);
 
 // CCCS over the forward cones (DoseComp.cu:217-392): per-cone partial BEV doses
@@ -23,14 +29,23 @@ hipError_t launch_cccs(const BeamletDev* beams, int n, int max_L, const ConeTabl
 hipError_t launch_resample_count(const BeamletDev* beams, int n, const fd_volume& vol, uint32_t* cand_count,
                                  hipStream_t stream);
 // Stage 2: values of every candidate, nonzeros compacted in voxel order into tmp_cols/tmp_vals
-// at cand_off[b * Z + z]; nonzero counts into nnz_count.
+// at cand_off[b * Z + z]; nonzero counts into nnz_count.  A segment whose candidates would end
+// beyond `cap` entries writes nothing, gets a zero count and sets overflow[0] = 1.
+// rowmax_bits (nullable, zeroed by the caller): per-beamlet maximum value, as float bits.
 hipError_t launch_resample_values(const BeamletDev* beams, int n, const fd_volume& vol, float extent,
-                                  const float* dose_bev, const uint64_t* cand_off, uint64_t* tmp_cols,
-                                  float* tmp_vals, uint32_t* nnz_count, hipStream_t stream);
-// Stage 3: copy the compacted nonzeros into the CSR arrays; row offsets from nnz_off.
+                                  const float* dose_bev, const uint64_t* cand_off, uint64_t cap, uint32_t* overflow,
+                                  uint64_t* tmp_cols, float* tmp_vals, uint32_t* nnz_count, uint32_t* rowmax_bits,
+                                  hipStream_t stream);
+// Relative threshold: per segment, the entries >= thr * rowmax of its beamlet.
+hipError_t launch_resample_threshold_count(int n, int Z, const uint64_t* cand_off, const uint32_t* nnz_count,
+                                           const float* tmp_vals, const uint32_t* rowmax_bits, float thr,
+                                           uint32_t* keep_count, hipStream_t stream);
+// Stage 3: copy the compacted nonzeros into the CSR arrays (with rowmax_bits and thr > 0 only
+// those >= thr * rowmax); row offsets from nnz_off.
 hipError_t launch_resample_compact(int n, int Z, const uint64_t* cand_off, const uint64_t* nnz_off,
                                    const uint32_t* nnz_count, const uint64_t* tmp_cols, const float* tmp_vals,
-                                   uint64_t* offsets, uint64_t* columns, float* values, hipStream_t stream);
+                                   const uint32_t* rowmax_bits, float thr, uint64_t* offsets, uint64_t* columns,
+                                   float* values, hipStream_t stream);
 
 // Exclusive scan of count[0..N) (u32) into off[0..N) (u64) plus off_total[0] = sum.
 hipError_t launch_scan_u32(const uint32_t* count, uint64_t N, uint64_t* off, uint64_t* total,
@@ -42,7 +57,25 @@ hipError_t launch_dense_count(const float* dense, uint64_t rows, uint64_t cols, 
 hipError_t launch_dense_fill(const float* dense, uint64_t rows, uint64_t cols, const uint64_t* row_off,
                              uint64_t* columns, float* values, hipStream_t stream);
 // offsets[r] = row_off[r] (r < rows), offsets[rows] = total[0].
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t stream);
+hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t stream);
 hipError_t launch_write_offsets(const uint64_t* row_off, const uint64_t* total, uint64_t rows, uint64_t* offsets,
                                 hipStream_t stream);
+
+// Packing and D^T X products (kernels_spmm.hip).
+hipError_t launch_global_rows(int n_mat, const uint64_t* row_first, const uint64_t* off_off, const uint64_t* nnz_off,
+                              const uint64_t* offsets, uint64_t R, uint64_t total_nnz, uint64_t* rowptr,
+                              hipStream_t stream);
+// scale[r] = row maximum; q (nullable) = value / scale in BF16 or FP16
+hipError_t launch_pack_rows(uint64_t R, const uint64_t* rowptr, const float* values, int vtype, float* scale,
+                            uint16_t* q, hipStream_t stream);
+hipError_t launch_col_count(uint64_t nnz, const uint64_t* columns, uint32_t* count, hipStream_t stream);
+hipError_t launch_col_scatter(uint64_t R, const uint64_t* rowptr, const uint64_t* columns, const void* vals,
+                              int vbytes, const uint64_t* colptr, uint32_t* cursor, uint32_t* crow, void* cval,
+                              hipStream_t stream);
+hipError_t launch_col_sort(uint64_t C, const uint64_t* colptr, uint32_t* crow, void* cval, int vbytes,
+                           hipStream_t stream);
+hipError_t launch_spmm(int vtype, uint64_t C, const uint64_t* colptr, const uint32_t* crow, const void* cval,
+                       const float* scale, const float* X, int k, float* Y, hipStream_t stream);
 
 }  // namespace fdi

@@ -4,6 +4,7 @@
 // (IMRTOpt/src/IMRTSparseMatEns.cu:78-161): row-major input, exact zeros dropped, ascending
 // columns, 0-based 64-bit offsets.  Count pass (one workgroup per row) -> scan -> ordered fill
 // (one workgroup per row, wavefront ballots keep the column order).
+#include <algorithm>
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -108,6 +109,28 @@ hipError_t launch_dense_fill(const float* dense, uint64_t rows, uint64_t cols, c
     if (rows == 0) return hipSuccess;
     hipLaunchKernelGGL(dense_fill_kernel, dim3((unsigned)rows), dim3(256), 0, stream, dense, cols, row_off, columns,
                        values);
+    return hipGetLastError();
+}
+
+// Plain fill kernels: clearing through a kernel rather than a memset node keeps every stage
+// capturable into a HIP graph.
+template <typename T>
+__global__ void fill_kernel(T* p, uint64_t n, T v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3((unsigned)blocks), dim3(256), 0, stream, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(fill_kernel<uint64_t>, dim3((unsigned)blocks), dim3(256), 0, stream, p, n, v);
     return hipGetLastError();
 }
 

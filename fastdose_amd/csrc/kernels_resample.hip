@@ -194,8 +194,9 @@ __device__ float voxel_value(const BeamletDev& B, const float* __restrict__ dose
 
 __global__ __launch_bounds__(kRowThreads) void resample_values_kernel(
     const BeamletDev* __restrict__ beams, int X, int Y, int Z, float vsx, float vsy, float vsz, float extent,
-    const float* __restrict__ dose, const uint64_t* __restrict__ cand_off, uint64_t* __restrict__ tmp_cols,
-    float* __restrict__ tmp_vals, uint32_t* __restrict__ nnz_count) {
+    const float* __restrict__ dose, const uint64_t* __restrict__ cand_off, uint64_t cap,
+    uint32_t* __restrict__ overflow, uint64_t* __restrict__ tmp_cols, float* __restrict__ tmp_vals,
+    uint32_t* __restrict__ nnz_count, uint32_t* __restrict__ rowmax_bits) {
     extern __shared__ uint32_t s_rows[];  // x0[Y], off[Y + 1]
     __shared__ uint32_t s_wave[kRowThreads / 64 + 1];
     uint32_t* s_x0 = s_rows;
@@ -224,12 +225,20 @@ __global__ __launch_bounds__(kRowThreads) void resample_values_kernel(
     __syncthreads();
     const uint32_t total = carry;
     const uint64_t base = cand_off[(size_t)b * Z + z];
+    if (base + total > cap) {  // staging too small: report, write nothing (the caller re-runs)
+        if (threadIdx.x == 0) {
+            overflow[0] = 1u;
+            nnz_count[(size_t)b * Z + z] = 0u;
+        }
+        return;
+    }
 
     const float half_diag = sqrtf(vsx * vsx + vsy * vsy + vsz * vsz) * 0.5f;
     const float svx = 8 * vsx, svy = 8 * vsy, svz = 8 * vsz;
     const float half_sdiag = sqrtf(svx * svx + svy * svy + svz * svz) * 0.5f;
 
     uint32_t written = 0;
+    float vmax = 0.0f;
     for (uint32_t c0 = 0; c0 < total; c0 += blockDim.x) {
         const uint32_t c = c0 + threadIdx.x;
         float v = 0.0f;
@@ -246,6 +255,7 @@ __global__ __launch_bounds__(kRowThreads) void resample_values_kernel(
             v = voxel_value(B, dose, x, y, z, vsx, vsy, vsz, extent, half_diag, half_sdiag);
         }
         const uint32_t nz = (c < total && v != 0.0f) ? 1u : 0u;
+        vmax = fmaxf(vmax, v);
         uint32_t tot;
         const uint32_t pos = block_exclusive_scan(nz, s_wave, tot);
         if (nz) {
@@ -256,6 +266,56 @@ __global__ __launch_bounds__(kRowThreads) void resample_values_kernel(
         written += tot;
     }
     if (threadIdx.x == 0) nnz_count[(size_t)b * Z + z] = written;
+    if (rowmax_bits) {
+        // beamlet maximum for the relative threshold (values are >= 0, so their bit patterns order
+        // like the floats)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
+        if ((threadIdx.x & 63) == 0 && vmax > 0.0f) atomicMax(&rowmax_bits[b], __float_as_uint(vmax));
+    }
+}
+
+// Entries of segment (z, b) at or above thr * (beamlet maximum).
+__global__ __launch_bounds__(256) void resample_threshold_count_kernel(int Z, const uint64_t* __restrict__ cand_off,
+                                                                       const uint32_t* __restrict__ nnz_count,
+                                                                       const float* __restrict__ tmp_vals,
+                                                                       const uint32_t* __restrict__ rowmax_bits,
+                                                                       float thr, uint32_t* __restrict__ keep_count) {
+    __shared__ uint32_t s_wave[256 / 64 + 1];
+    const size_t seg = (size_t)blockIdx.y * Z + blockIdx.x;
+    const float cut = thr * __uint_as_float(rowmax_bits[blockIdx.y]);
+    const uint32_t cnt = nnz_count[seg];
+    const uint64_t src = cand_off[seg];
+    uint32_t k = 0;
+    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) k += tmp_vals[src + i] >= cut ? 1u : 0u;
+    uint32_t total;
+    block_exclusive_scan(k, s_wave, total);
+    if (threadIdx.x == 0) keep_count[seg] = total;
+}
+
+// Ordered copy of the entries that pass the threshold into the CSR arrays.
+__global__ __launch_bounds__(256) void resample_compact_threshold_kernel(
+    int Z, const uint64_t* __restrict__ cand_off, const uint64_t* __restrict__ nnz_off,
+    const uint32_t* __restrict__ nnz_count, const uint64_t* __restrict__ tmp_cols, const float* __restrict__ tmp_vals,
+    const uint32_t* __restrict__ rowmax_bits, float thr, uint64_t* __restrict__ columns, float* __restrict__ values) {
+    __shared__ uint32_t s_wave[256 / 64 + 1];
+    const size_t seg = (size_t)blockIdx.y * Z + blockIdx.x;
+    const float cut = thr * __uint_as_float(rowmax_bits[blockIdx.y]);
+    const uint32_t cnt = nnz_count[seg];
+    const uint64_t src = cand_off[seg], dst = nnz_off[seg];
+    uint32_t written = 0;
+    for (uint32_t c0 = 0; c0 < cnt; c0 += blockDim.x) {
+        const uint32_t i = c0 + threadIdx.x;
+        const float v = i < cnt ? tmp_vals[src + i] : 0.0f;
+        const uint32_t keep = (i < cnt && v >= cut) ? 1u : 0u;
+        uint32_t tot;
+        const uint32_t pos = block_exclusive_scan(keep, s_wave, tot);
+        if (keep) {
+            columns[dst + written + pos] = tmp_cols[src + i];
+            values[dst + written + pos] = v;
+        }
+        written += tot;
+    }
 }
 
 __global__ __launch_bounds__(256) void resample_compact_kernel(int Z, const uint64_t* __restrict__ cand_off,
@@ -293,24 +353,39 @@ hipError_t launch_resample_count(const BeamletDev* beams, int n, const fd_volume
 }
 
 hipError_t launch_resample_values(const BeamletDev* beams, int n, const fd_volume& vol, float extent,
-                                  const float* dose_bev, const uint64_t* cand_off, uint64_t* tmp_cols,
-                                  float* tmp_vals, uint32_t* nnz_count, hipStream_t stream) {
+                                  const float* dose_bev, const uint64_t* cand_off, uint64_t cap, uint32_t* overflow,
+                                  uint64_t* tmp_cols, float* tmp_vals, uint32_t* nnz_count, uint32_t* rowmax_bits,
+                                  hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     dim3 grid(vol.dim[2], n), block(kRowThreads);
     const size_t shm = (2 * (size_t)vol.dim[1] + 1) * sizeof(uint32_t);
     hipLaunchKernelGGL(resample_values_kernel, grid, block, shm, stream, beams, (int)vol.dim[0], (int)vol.dim[1],
                        (int)vol.dim[2], vol.voxel_size[0], vol.voxel_size[1], vol.voxel_size[2], extent, dose_bev,
-                       cand_off, tmp_cols, tmp_vals, nnz_count);
+                       cand_off, cap, overflow, tmp_cols, tmp_vals, nnz_count, rowmax_bits);
+    return hipGetLastError();
+}
+
+hipError_t launch_resample_threshold_count(int n, int Z, const uint64_t* cand_off, const uint32_t* nnz_count,
+                                           const float* tmp_vals, const uint32_t* rowmax_bits, float thr,
+                                           uint32_t* keep_count, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(resample_threshold_count_kernel, dim3(Z, n), dim3(256), 0, stream, Z, cand_off, nnz_count,
+                       tmp_vals, rowmax_bits, thr, keep_count);
     return hipGetLastError();
 }
 
 hipError_t launch_resample_compact(int n, int Z, const uint64_t* cand_off, const uint64_t* nnz_off,
                                    const uint32_t* nnz_count, const uint64_t* tmp_cols, const float* tmp_vals,
-                                   uint64_t* offsets, uint64_t* columns, float* values, hipStream_t stream) {
+                                   const uint32_t* rowmax_bits, float thr, uint64_t* offsets, uint64_t* columns,
+                                   float* values, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     dim3 grid(Z, n), block(256);
-    hipLaunchKernelGGL(resample_compact_kernel, grid, block, 0, stream, Z, cand_off, nnz_off, nnz_count, tmp_cols,
-                       tmp_vals, columns, values);
+    if (rowmax_bits && thr > 0.0f)
+        hipLaunchKernelGGL(resample_compact_threshold_kernel, grid, block, 0, stream, Z, cand_off, nnz_off, nnz_count,
+                           tmp_cols, tmp_vals, rowmax_bits, thr, columns, values);
+    else
+        hipLaunchKernelGGL(resample_compact_kernel, grid, block, 0, stream, Z, cand_off, nnz_off, nnz_count, tmp_cols,
+                           tmp_vals, columns, values);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // nnz_off has n*Z entries followed by the total at index n*Z

@@ -45,12 +45,14 @@ __device__ __forceinline__ float trilinear(const float* __restrict__ vol, int X,
     return fmaf(az, c1 - c0, c0);
 }
 
-template <bool RPL64>
+// rpl64_len: beamlets whose central-axis path (long_dim * long_spacing) exceeds this many cm
+// accumulate the radiological path in FP64 (0: every beamlet; infinity: none).
 __global__ __launch_bounds__(256) void terma_kernel(const BeamletDev* __restrict__ beams,
                                                     const float* __restrict__ fluence,
                                                     const float* __restrict__ density, int X, int Y, int Z,
                                                     float vsx, float vsy, float vsz, SpectrumDev spec,
-                                                    float* __restrict__ terma, float* __restrict__ dens_bev) {
+                                                    float rpl64_len, float* __restrict__ terma,
+                                                    float* __restrict__ dens_bev) {
     const int b = blockIdx.x;
     const int p = threadIdx.x;
     const BeamletDev& B = beams[b];
@@ -75,6 +77,7 @@ __global__ __launch_bounds__(256) void terma_kernel(const BeamletDev* __restrict
 
     float* __restrict__ T = terma + B.bev_off + p;
     float* __restrict__ R = dens_bev + B.bev_off + p;
+    const bool RPL64 = (float)B.L * B.ls > rpl64_len;  // uniform over the workgroup
     float rpl = 0.0f;
     double rpl64 = 0.0;
     int k = 1;
@@ -107,17 +110,13 @@ __global__ __launch_bounds__(256) void terma_kernel(const BeamletDev* __restrict
 }  // namespace
 
 hipError_t launch_terma(const BeamletDev* beams, int n, const float* fluence, const float* density,
-                        const fd_volume& vol, const SpectrumDev& spec, bool rpl_fp64, float* terma,
+                        const fd_volume& vol, const SpectrumDev& spec, float rpl64_len, float* terma,
                         float* dens_bev, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     dim3 grid(n), block(kNpix);
     const int X = vol.dim[0], Y = vol.dim[1], Z = vol.dim[2];
-    if (rpl_fp64)
-        hipLaunchKernelGGL(terma_kernel<true>, grid, block, 0, stream, beams, fluence, density, X, Y, Z,
-                           vol.voxel_size[0], vol.voxel_size[1], vol.voxel_size[2], spec, terma, dens_bev);
-    else
-        hipLaunchKernelGGL(terma_kernel<false>, grid, block, 0, stream, beams, fluence, density, X, Y, Z,
-                           vol.voxel_size[0], vol.voxel_size[1], vol.voxel_size[2], spec, terma, dens_bev);
+    hipLaunchKernelGGL(terma_kernel, grid, block, 0, stream, beams, fluence, density, X, Y, Z, vol.voxel_size[0],
+                       vol.voxel_size[1], vol.voxel_size[2], spec, rpl64_len, terma, dens_bev);
     return hipGetLastError();
 }
 

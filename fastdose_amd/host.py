@@ -17,7 +17,7 @@ from pathlib import Path
 import numpy as np
 
 from . import _lib
-from ._lib import Beam, Bundle, CSR, Kernel, Spectrum, Volume, call, load
+from ._lib import Beam, Bundle, CSR, Kernel, ServeOpts, ServeStats, Spectrum, Volume, call, load
 
 NPIX = 256
 
@@ -124,6 +124,10 @@ class Context:
     def set_options(self, options: int) -> None:
         call("fd_ctx_set_options", self._h, options)
 
+    def set_threshold(self, rel: float) -> None:
+        """Keep CSR entries >= rel x (the beamlet's maximum); 0 drops exact zeros only."""
+        call("fd_ctx_set_threshold", self._h, rel)
+
     def timings(self) -> dict:
         """Per-kernel milliseconds of the last launches (needs FD_OPT_TIMING)."""
         n = len(_lib.TIMING_SLOTS)
@@ -147,6 +151,19 @@ class Context:
 
     def __exit__(self, *exc):
         self.close()
+
+    def beamlet_flags(self, mask_dev, bundles, stream=None) -> list:
+        """d_beamletFlagInit on the device for many bundles; mask_dev: u8 device tensor (z, y, x)."""
+        arr = (Bundle * len(bundles))(*bundles)
+        sizes = [b.fluence_dim[0] * b.fluence_dim[1] for b in bundles]
+        out = np.zeros(max(sum(sizes), 1), dtype=np.uint8)
+        call("fd_beamlet_flags", self._h, C.c_void_p(mask_dev.data_ptr()), arr, len(bundles), _u8(out),
+             _stream_arg(stream))
+        res, o = [], 0
+        for n in sizes:
+            res.append(out[o:o + n].copy())
+            o += n
+        return res
 
     # ---- split stages on torch device tensors
     def terma(self, beams, fluence_dev, terma_dev, dens_dev, stride: int, stream=None) -> None:
@@ -263,6 +280,19 @@ class DoseMat:
              col.ctypes.data_as(C.POINTER(C.c_uint64)), _f32(val))
         return off, col[: nnz.value], val[: nnz.value]
 
+    def pack(self, value_type: int, stream=None) -> None:
+        """16-bit values (FD_VAL_BF16 / FD_VAL_FP16) with a per-row FP32 scale; tofile then also writes
+        valuesBuffer.{bf16,fp16}.bin and beamletScale.bin."""
+        call("fd_dosemat_pack", self._h, value_type, _stream_arg(stream))
+
+    def packed(self):
+        """(per-row scales f32, packed values u16) of the last pack()."""
+        _, rows, nnz = self.info()
+        scale = np.zeros(max(rows, 1), dtype=np.float32)
+        q = np.zeros(max(nnz, 1), dtype=np.uint16)
+        call("fd_dosemat_copy_packed", self._h, _f32(scale), q.ctypes.data_as(C.POINTER(C.c_uint16)))
+        return scale[:rows], q[:nnz]
+
     def tofile(self, folder) -> None:
         call("fd_dosemat_tofile", self._h, str(folder).encode())
 
@@ -322,3 +352,116 @@ def dose_mat_construction(ctx: Context, bundles, flags_list, extent: float = 2.0
         beams, flu = beamlets_init(bundle, flags, ctx.vol)
         dm.add_beam(ctx, beams, flu, extent, stream)
     return dm
+
+
+class Spmm:
+    """Voxel-major copy of a DoseMat for Y = D^T X (fluence vectors -> dose), FP32 accumulation."""
+
+    def __init__(self, dm: DoseMat, value_type: int = 0, stream=None):
+        h = C.c_void_p()
+        call("fd_spmm_create", dm._h, value_type, C.byref(h), _stream_arg(stream))
+        self._h = h
+
+    def info(self):
+        r, c, n = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        call("fd_spmm_info", self._h, C.byref(r), C.byref(c), C.byref(n))
+        return r.value, c.value, n.value
+
+    def run(self, x_dev, y_dev=None, stream=None):
+        """x_dev: (rows, k) float32 device tensor; returns y (n_cols, k)."""
+        import torch
+
+        rows, cols, _ = self.info()
+        if x_dev.dtype != torch.float32 or not x_dev.is_contiguous() or x_dev.dim() != 2 or x_dev.shape[0] != rows:
+            raise ValueError("x must be a contiguous float32 (rows, k) tensor")
+        k = int(x_dev.shape[1])
+        if y_dev is None:
+            y_dev = torch.empty((cols, k), dtype=torch.float32, device=x_dev.device)
+        call("fd_spmm_run", self._h, C.c_void_p(x_dev.data_ptr()), k, C.c_void_p(y_dev.data_ptr()),
+             _stream_arg(stream))
+        return y_dev
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            load().fd_spmm_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Server:
+    """Resident request loop with continuous batching over a Context (fd_serve_*).
+
+    submit() queues a request of one or more beams and returns its id at once; slot threads pack
+    queued beams into batches (admitted by their HBM footprint) that run concurrently on their own
+    streams; fetch() returns the request's per-beam matrices."""
+
+    def __init__(self, ctx: Context, n_slots: int = 3, max_batch_beamlets: int = 4096, hbm_budget: int = 0,
+                 extent: float = 2.0, options: int = 0, threshold: float = 0.0):
+        self.ctx = ctx
+        opts = ServeOpts(n_slots, max_batch_beamlets, hbm_budget, extent, options, threshold)
+        h = C.c_void_p()
+        call("fd_serve_start", ctx.handle, C.byref(opts), C.byref(h))
+        self._h = h
+
+    def submit(self, beamlets, fluence: np.ndarray, counts) -> int:
+        arr = beam_array(beamlets)
+        cnt = np.ascontiguousarray(counts, dtype=np.int32)
+        if int(cnt.sum()) != len(arr):
+            raise ValueError("counts must add up to the number of beamlets")
+        flu = np.ascontiguousarray(fluence, dtype=np.float32)
+        if flu.size != len(arr) * NPIX:
+            raise ValueError("fluence must hold 256 floats per beamlet")
+        rid = C.c_uint64()
+        call("fd_serve_submit", self._h, arr, _f32(flu), cnt.ctypes.data_as(C.POINTER(C.c_int)), len(cnt),
+             C.byref(rid))
+        return rid.value
+
+    def poll(self, rid: int) -> int:
+        st = C.c_int()
+        call("fd_serve_poll", self._h, rid, C.byref(st))
+        return st.value
+
+    def wait(self, rid: int, timeout: float = 0.0) -> None:
+        call("fd_serve_wait", self._h, rid, timeout)
+
+    def fetch(self, rid: int, into: "DoseMat | None" = None) -> "DoseMat":
+        vol = self.ctx.vol
+        dm = into if into is not None else DoseMat(vol.dim[0] * vol.dim[1] * vol.dim[2])
+        call("fd_serve_fetch", self._h, rid, dm._h)
+        return dm
+
+    def stats(self) -> dict:
+        st = ServeStats()
+        call("fd_serve_stats_get", self._h, C.byref(st))
+        return {k: getattr(st, k) for k, _ in ServeStats._fields_}
+
+    def reset_stats(self) -> None:
+        call("fd_serve_stats_reset", self._h)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            load().fd_serve_stop(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -45,8 +45,16 @@ enum {
     FD_OPT_EXACT_G = 1,      /* use g = (1-e^-x)/x (KernelGen/derivation.md:248) instead of the
                                 reference's g == 1 (DoseComp.cu:167-175) */
     FD_OPT_RPL_FP64 = 2,     /* accumulate the Terma radiological path in FP64 on rays longer
-                                than 256 voxels */
-    FD_OPT_TIMING = 4        /* record HIP events around every kernel launch (fd_ctx_timings) */
+                                than 256 voxels (long_dim * long_spacing > 256 * min voxel size) */
+    FD_OPT_TIMING = 4,       /* record HIP events around every kernel launch (fd_ctx_timings) */
+    FD_OPT_RPL_FP64_ALL = 8  /* FP64 radiological path on every ray */
+};
+
+/* Value types of packed dose-loading matrices (fd_dosemat_pack, fd_spmm_create). */
+enum {
+    FD_VAL_F32 = 0,  /* the canonical FP32 values */
+    FD_VAL_BF16 = 1, /* bfloat16 of value / row scale (row scale = the beamlet's maximum, FP32) */
+    FD_VAL_FP16 = 2  /* IEEE half of value / row scale */
 };
 
 /* Kernel slots reported by fd_ctx_timings. */
@@ -160,6 +168,14 @@ int fd_ptv_isocenter(const uint8_t* mask, const fd_volume* vol, float iso[3]);
 int fd_beamlet_flags_host(const fd_bundle* bundle, const uint8_t* ptv_mask, const fd_volume* vol,
                           uint8_t* flags);
 
+/* beamletFlagInit / d_beamletFlagInit (IMRTBeamBundle.cu:130-316) on the device, for n_bundles
+ * bundles at once: mask_dev is the u8 PTV mask of the context's volume in device memory (x
+ * fastest); flags_host receives fluence_dim^2 flags per bundle, concatenated in bundle order.
+ * Bit-identical to fd_beamlet_flags_host; a bundle whose central beam misses the volume gets all-zero
+ * flags (fd_beamlet_flags_host reports FD_ERR_GEOM for it). */
+int fd_beamlet_flags(fd_ctx* ctx, const uint8_t* mask_dev, const fd_bundle* bundles, int n_bundles,
+                     uint8_t* flags_host, void* stream);
+
 /* SPECTRUM_h::read_spectrum_file (spectrum.cu:14-60). */
 int fd_read_spectrum(const char* path, fd_spectrum* out);
 /* KERNEL_h::read_kernel_file (kernel.cu:13-72). */
@@ -173,6 +189,10 @@ int fd_ctx_create(int device, const fd_volume* vol, const float* density_host, c
                   const fd_kernel* kernel, fd_ctx** out);
 int fd_ctx_destroy(fd_ctx* ctx);
 int fd_ctx_set_options(fd_ctx* ctx, int options);
+/* Relative CSR threshold: the resampler keeps only entries >= rel * (the beamlet's maximum value).
+ * 0 (default) drops exact zeros only, as cuSPARSE DenseToSparse in MatCSREnsemble::addMat
+ * (IMRTSparseMatEns.cu:108-161) does; the reference has no threshold. */
+int fd_ctx_set_threshold(fd_ctx* ctx, float rel);
 /* Milliseconds each kernel slot took in the last call that launched it (FD_OPT_TIMING set),
  * measured with HIP events on the launch stream; -1 for slots not run.  Plays the role of the
  * reference's cudaEvent timers (TermaComp.cu:310-333, DoseComp.cu:449-472). */
@@ -235,8 +255,75 @@ int fd_dosemat_tofile(const fd_dosemat* m, const char* folder);
 /* MatCSREnsemble::fromfile (IMRTSparseMatEns.cu:242-336). */
 int fd_dosemat_fromfile(fd_dosemat* m, const char* folder);
 
+/* 16-bit copy of the ensemble's values: per row (beamlet, all matrices in order) an FP32 scale =
+ * the row maximum, and q = value / scale rounded to nearest even in BF16 or FP16.  After it,
+ * fd_dosemat_tofile also writes valuesBuffer.bf16.bin or valuesBuffer.fp16.bin (u16 per entry,
+ * entry order of valuesBuffer.bin) and beamletScale.bin (f32 per row); the canonical files stay as
+ * readDose.m reads them.  The reference stores FP32 only (IMRTSparseMatEns.cu:225-236). */
+int fd_dosemat_pack(fd_dosemat* m, int value_type, void* stream);
+/* Host copy of the packed data (rows_total scales, nnz_total u16 values). */
+int fd_dosemat_copy_packed(const fd_dosemat* m, float* scale, uint16_t* values);
+
 /* beamletFlagSave (IMRTBeamBundle.cu:572-601): u8 flags, n_beams * per_beam. */
 int fd_write_fluence_map(const char* path, const uint8_t* flags, uint64_t n_beams, uint64_t per_beam);
+
+/* ---- D^T X: fluence vectors through the dose-loading matrix ----------------------------------
+ * The optimiser multiplies the matrix by fluence vectors (IMRTOptimize.cu:604-625, cuSPARSE on the
+ * FP32 CSR; IMRTDebug.cu:211-315).  fd_spmm_create builds a voxel-major copy of an ensemble (all
+ * matrices, rows in order) with FP32, BF16 or FP16 values (16-bit: per-row FP32 scale, as
+ * fd_dosemat_pack); fd_spmm_run computes Y = D^T X for k right-hand sides with FP32 accumulation:
+ * x_dev is rows_total x k (row-major), y_dev is n_cols x k (row-major).  Each column's entries are
+ * summed in row order, so results are bitwise reproducible. */
+typedef struct fd_spmm fd_spmm;
+int fd_spmm_create(const fd_dosemat* m, int value_type, fd_spmm** out, void* stream);
+int fd_spmm_info(const fd_spmm* p, uint64_t* rows, uint64_t* cols, uint64_t* nnz);
+int fd_spmm_run(fd_spmm* p, const float* x_dev, int k, float* y_dev, void* stream);
+int fd_spmm_destroy(fd_spmm* p);
+
+/* ---- Serving: a resident request loop over one context ------------------------------------
+ * The reference computes beams one at a time inside DoseMatConstruction (IMRTDoseMat.cu:183-268,
+ * one beam per TermaComputeCollective / DoseComputeCollective / BEV2PVCSInterp round with a host
+ * synchronisation after each); it has no request loop.  These entry points keep the context's
+ * density, spectrum and cone tables resident and take dose requests (one or more beams, any
+ * beamlet count) as they arrive.  Slot threads, each with its own stream and scratch, pull beams
+ * from the queue in arrival order, pack them into batches of up to max_batch_beamlets, and admit
+ * a batch only while the device scratch of all batches in flight stays within hbm_budget, so
+ * requests that arrive while others run are admitted as slots and memory free up and their work
+ * overlaps on the device.  Each beam's matrix is identical to fd_dosemat_add_beam's for it alone. */
+typedef struct fd_server fd_server;
+
+typedef struct fd_serve_opts {
+    int n_slots;                 /* concurrent batches (streams); 0: 3 */
+    int max_batch_beamlets;      /* beamlets per batch; 0: 4096 */
+    uint64_t hbm_budget;         /* bytes of batch scratch in flight; 0: half the free HBM */
+    float extent;                /* resampler extent [cm]; 0: 2.0 */
+    int options;                 /* FD_OPT_EXACT_G / FD_OPT_RPL_FP64(_ALL) for the batches */
+    float threshold;             /* relative CSR threshold, as fd_ctx_set_threshold */
+} fd_serve_opts;
+
+typedef struct fd_serve_stats {
+    uint64_t requests_done, beams_done, beamlets_done, batches;
+    double p50_ms, p99_ms, mean_ms, max_ms; /* per-beam latency: request submit -> beam ready */
+    double first_submit_s, last_done_s;     /* seconds since fd_serve_start */
+    uint64_t peak_inflight_bytes;           /* admitted scratch, high-water mark */
+    int peak_inflight_batches;
+} fd_serve_stats;
+
+int fd_serve_start(fd_ctx* ctx, const fd_serve_opts* opts, fd_server** out);
+/* Copies the request (beamlets of n_beams beams concatenated, counts[k] each, 256 fluence floats
+ * per beamlet) and queues it; returns at once with its id. */
+int fd_serve_submit(fd_server* s, const fd_beam* beamlets, const float* fluence, const int* counts, int n_beams,
+                    uint64_t* request_id);
+/* *state: 0 queued or running, 1 finished, -1 failed (fd_serve_fetch reports the error). */
+int fd_serve_poll(fd_server* s, uint64_t request_id, int* state);
+/* Blocks until the request finishes or timeout_s passes (FD_ERR_CAPACITY on time-out). */
+int fd_serve_wait(fd_server* s, uint64_t request_id, double timeout_s);
+/* Waits, appends the request's beams to m as matrices (in beam order) and forgets the request. */
+int fd_serve_fetch(fd_server* s, uint64_t request_id, fd_dosemat* m);
+int fd_serve_stats_get(fd_server* s, fd_serve_stats* out);
+int fd_serve_stats_reset(fd_server* s);
+/* Finishes the queued work, joins the slot threads and frees the server (not the context). */
+int fd_serve_stop(fd_server* s);
 
 #ifdef __cplusplus
 }

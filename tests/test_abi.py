@@ -39,6 +39,8 @@ def test_struct_layouts_match_header():
     assert C.sizeof(fd.Kernel) == 8 + 8 * 4 * 24
     assert C.sizeof(fd.Bundle) == 4 * 16
     assert C.sizeof(fd.CSR) == 48
+    assert C.sizeof(fd._lib.ServeOpts) == 32
+    assert C.sizeof(fd._lib.ServeStats) == 4 * 8 + 6 * 8 + 8 + 8
 
 
 def test_version_and_error_strings():

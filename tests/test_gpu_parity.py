@@ -10,7 +10,7 @@ import fastdose_amd as fd
 import oracle as orc
 from fastdose_amd import synth
 
-from gpu_util import beamlets_of, dense_diff, gpu_stages, log, oracle_beamlet, row
+from gpu_util import beamlets_of, dense_diff, gamma_pass_rate, gpu_stages, log, oracle_beamlet, row
 
 pytestmark = pytest.mark.gpu
 
@@ -27,7 +27,7 @@ def torch_cuda():
 
 
 def check_against_oracle(cfg, beams, flu, g, idx, flags=0, stage_isolated=True):
-    worst = {"terma": 0.0, "dose": 0.0, "csr": 0.0}
+    worst = {"terma": 0.0, "dose": 0.0, "csr": 0.0, "gamma_fail": 0.0}
     for i in idx:
         b = beams[i]
         L = b.long_dim
@@ -46,6 +46,9 @@ def check_against_oracle(cfg, beams, flu, g, idx, flags=0, stage_isolated=True):
         assert et < 1e-4 and er < 1e-4, (i, et, er)
         assert ed <= TOL, (i, ed)
         assert ec <= TOL, (i, ec)
+        gp = gamma_pass_rate(cfg.volume, ref["cols"], ref["vals"], cols, vals, dd=0.01, dta=0.1)
+        worst["gamma_fail"] = max(worst["gamma_fail"], 1.0 - gp)
+        assert gp >= 0.99, (i, gp)  # gamma 1 %/1 mm
         # index sets: identical except for entries that are structurally ~0 on one side
         if len(sym):
             u = np.union1d(cols, ref["cols"])
@@ -214,6 +217,47 @@ def test_batched_beams_match_per_beam(torch_cuda):
                 a, b = ref.matrix(k), dm.matrix(k)
                 assert all(np.array_equal(x, y) for x, y in zip(a, b)), (chunk, k)
         assert ref.matrix(2)[0].tolist() == [0]
+
+
+def test_device_beamlet_flags_match_host(torch_cuda):
+    """fd_beamlet_flags (device, many bundles) == fd_beamlet_flags_host, bit for bit."""
+    torch = torch_cuda
+    cfg = synth.fourpi_config("flags-4pi", (60, 56, 48), 1011, 24, (7.0, 5.0, 4.0), fluence_dim=12,
+                              with_flags=False)
+    segs = cfg.notes["segments"]
+    seg_of = cfg.notes["segment_of_beam"]
+    with fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel) as ctx:
+        for s_idx, mask in enumerate(segs):
+            ks = [k for k in range(len(cfg.bundles)) if seg_of[k] == s_idx]
+            bundles = [cfg.bundles[k] for k in ks]
+            mask_d = torch.from_numpy(np.ascontiguousarray(mask, dtype=np.uint8)).cuda()
+            dev = ctx.beamlet_flags(mask_d, bundles)
+            for b, f in zip(bundles, dev):
+                host = fd.beamlet_flag_init(b, mask, cfg.volume)
+                assert np.array_equal(host, f)
+        # a bundle far outside the volume: all flags 0
+        far = synth.make_bundle(np.array([500.0, 500.0, 500.0], np.float32), (0.0, 0.0, 0.0), fluence_dim=4)
+        mask_d = torch.from_numpy(np.ascontiguousarray(segs[0], dtype=np.uint8)).cuda()
+        assert not ctx.beamlet_flags(mask_d, [far])[0].any()
+
+
+def test_rpl_fp64_rule(torch_cuda):
+    """FD_OPT_RPL_FP64 uses FP64 only on rays longer than 256 voxels (none in this volume, so the
+    output is the FP32 one bit for bit); FD_OPT_RPL_FP64_ALL uses it everywhere and stays within the
+    FP32 tolerance of the oracle."""
+    cfg = synth.small_config(seed=14, dim=(24, 24, 24), fluence_dim=2)
+    beams, flu = beamlets_of(cfg)
+    outs = {}
+    for opt in (0, fd.FD_OPT_RPL_FP64, fd.FD_OPT_RPL_FP64_ALL):
+        with fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel, options=opt) as ctx:
+            outs[opt] = gpu_stages(ctx, beams, flu, cfg.extent)
+    assert np.array_equal(outs[0]["terma"], outs[fd.FD_OPT_RPL_FP64]["terma"])
+    g = outs[fd.FD_OPT_RPL_FP64_ALL]
+    for i in range(len(beams)):
+        L = beams[i].long_dim
+        ref = oracle_beamlet(cfg, beams[i], flu[i])
+        et = np.abs(g["terma"][i, :L] - ref["terma"]).max() / ref["terma"].max()
+        assert et < 1e-4, (i, et)
 
 
 def test_error_paths(torch_cuda):
