@@ -63,6 +63,212 @@ def cpu_baseline(cfg, beams, flu, sample: int):
     }
 
 
+def device_flags(ctx, cfg, torch):
+    """Beamlet/PTV flags of every bundle of a 4pi config on the device (one call per PTV segment)."""
+    segs, seg_of = cfg.notes["segments"], cfg.notes["segment_of_beam"]
+    flags = [None] * len(cfg.bundles)
+    for s_idx, mask in enumerate(segs):
+        ks = [k for k in range(len(cfg.bundles)) if seg_of[k] == s_idx]
+        mask_d = torch.from_numpy(mask.astype("uint8", copy=False)).cuda()
+        for k, f in zip(ks, ctx.beamlet_flags(mask_d, [cfg.bundles[k] for k in ks])):
+            flags[k] = f
+    return flags
+
+
+class BeamStore:
+    """All beamlets of a set of beams, concatenated, for cheap request assembly."""
+
+    def __init__(self, fd, cfg, flags):
+        import ctypes as C
+
+        import numpy as np
+
+        per = [fd.beamlets_init(b, f, cfg.volume) for b, f in zip(cfg.bundles, flags)]
+        self.counts = np.array([len(b) for b, _ in per], dtype=np.int64)
+        self.first = np.concatenate([[0], np.cumsum(self.counts)])
+        n = int(self.first[-1])
+        self.beams = (fd.Beam * n)()
+        for k, (b, _) in enumerate(per):
+            C.memmove(C.byref(self.beams, int(self.first[k]) * C.sizeof(fd.Beam)), b, C.sizeof(fd.Beam) * len(b))
+        self.flu = np.concatenate([f for _, f in per], 0)
+        self.fd = fd
+
+    def request(self, ks):
+        import ctypes as C
+
+        import numpy as np
+
+        cnt = [int(self.counts[k]) for k in ks]
+        arr = (self.fd.Beam * sum(cnt))()
+        o = 0
+        for k, c in zip(ks, cnt):
+            C.memmove(C.byref(arr, o * C.sizeof(self.fd.Beam)),
+                      C.byref(self.beams, int(self.first[k]) * C.sizeof(self.fd.Beam)), C.sizeof(self.fd.Beam) * c)
+            o += c
+        flu = np.concatenate([self.flu[self.first[k]:self.first[k] + self.counts[k]] for k in ks], 0)
+        return arr, flu, cnt
+
+
+def leg_batched_4pi(fd, synth, torch, np):
+    """Batched 4pi workload: 180 beams on a 200x200x160 body, 4 isocenters, 20x20 fluence, beamlets
+    that intersect the PTV; plain one-beam-at-a-time path vs all beams batched."""
+    t0 = time.perf_counter()
+    cfg = synth.fourpi_config("batched-4pi-200x200x160", (200, 200, 160), 1003, 180, (22.0, 15.0, 13.0),
+                              with_flags=False)
+    ctx = fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel)
+    flags = device_flags(ctx, cfg, torch)
+    store = BeamStore(fd, cfg, flags)
+    n = int(store.counts.sum())
+    nvox = cfg.volume.dim[0] * cfg.volume.dim[1] * cfg.volume.dim[2]
+    log(f"batched 4pi: {len(cfg.bundles)} beams, {n} beamlets, setup {time.perf_counter() - t0:.1f} s")
+    out = {"beams": len(cfg.bundles), "beamlets": n,
+           "beamlets_per_beam": [int(store.counts.min()), int(store.counts.max())]}
+    # warm both paths (allocations)
+    dm = fd.DoseMat(nvox)
+    dm.add_beams(ctx, *store.request(range(4))[:2], store.counts[:4], 0, cfg.extent)
+    dm.close()
+    # plain: one beam per launch
+    dm = fd.DoseMat(nvox)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    lat = []
+    for k in range(len(cfg.bundles)):
+        arr, flu, _ = store.request([k])
+        tb = time.perf_counter()
+        dm.add_beam(ctx, arr, flu, cfg.extent)
+        lat.append((time.perf_counter() - tb) * 1e3)
+    plain_s = time.perf_counter() - t
+    nnz_plain = dm.info()[2]
+    dm.close()
+    # batched: every beam through one launch per stage (chunks of 8192 beamlets)
+    arr, flu, cnt = store.request(range(len(cfg.bundles)))
+    dm = fd.DoseMat(nvox)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    dm.add_beams(ctx, arr, flu, cnt, 0, cfg.extent)
+    batched_s = time.perf_counter() - t
+    nnz_b = dm.info()[2]
+    dm.close()
+    ctx.close()
+    if nnz_b != nnz_plain:
+        raise RuntimeError("batched and plain paths disagree on the nonzero count")
+    out.update({
+        "plain_beamlets_per_s": round(n / plain_s, 1),
+        "plain_beam_latency_ms": {"p50": round(percentile(lat, 50), 3), "p99": round(percentile(lat, 99), 3)},
+        "batched_beamlets_per_s": round(n / batched_s, 1),
+        "batched_over_plain": round(plain_s / batched_s, 3),
+        "nnz": int(nnz_b),
+    })
+    log(f"batched 4pi: plain {n / plain_s:.0f} beamlets/s, batched {n / batched_s:.0f} beamlets/s")
+    return out, n / batched_s
+
+
+def leg_serving(fd, synth, torch, np, capacity_beamlets_s, n_req=30):
+    """Serving: 220x220x150 body with 1160 candidate 4pi beams; Poisson arrivals of 1-20-beam requests
+    at several rates; p50/p99 per-beam latency (submit -> beam ready) and beamlets/s."""
+    import threading
+
+    t0 = time.perf_counter()
+    cfg = synth.fourpi_config("serving-4pi-220x220x150", (220, 220, 150), 1005, 1160, (26.0, 18.0, 15.0),
+                              with_flags=False)
+    ctx = fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel)
+    flags = device_flags(ctx, cfg, torch)
+    store = BeamStore(fd, cfg, flags)
+    nvox = cfg.volume.dim[0] * cfg.volume.dim[1] * cfg.volume.dim[2]
+    log(f"serving: {len(cfg.bundles)} candidate beams, setup {time.perf_counter() - t0:.1f} s")
+    rng = np.random.default_rng(1005)
+    reqs = [list(rng.choice(len(cfg.bundles), size=int(rng.integers(1, 21)), replace=False)) for _ in range(n_req)]
+    payload = [store.request(ks) for ks in reqs]
+    total_beamlets = sum(sum(c) for _, _, c in payload)
+    mean_req = total_beamlets / n_req
+    out = {"candidate_beams": len(cfg.bundles), "requests_per_rate": n_req,
+           "beams_per_request": [1, 20], "beamlets_per_beam": [int(store.counts.min()), int(store.counts.max())],
+           "rates": []}
+    # plain one-beam-per-launch on the same beams (first 40 beams of the requests)
+    flat = [k for ks in reqs for k in ks][:40]
+    dm = fd.DoseMat(nvox)
+    dm.add_beam(ctx, *store.request([flat[0]])[:2], cfg.extent)
+    lat, nb = [], 0
+    t = time.perf_counter()
+    for k in flat:
+        arr, flu, c = store.request([k])
+        tb = time.perf_counter()
+        dm.add_beam(ctx, arr, flu, cfg.extent)
+        lat.append((time.perf_counter() - tb) * 1e3)
+        nb += c[0]
+    plain_s = time.perf_counter() - t
+    dm.close()
+    out["plain"] = {"beamlets_per_s": round(nb / plain_s, 1), "beam_latency_ms": {
+        "p50": round(percentile(lat, 50), 3), "p99": round(percentile(lat, 99), 3)}}
+    srv = fd.Server(ctx, n_slots=3, max_batch_beamlets=4096, extent=cfg.extent)
+    # warm-up request (allocations of every slot)
+    for w in range(3):
+        srv.fetch(srv.submit(*payload[w])).close()
+    for mult in (0.5, 0.9, 1.5):
+        rate = mult * capacity_beamlets_s / mean_req  # requests per second
+        gaps = rng.exponential(1.0 / rate, size=n_req)
+        srv.reset_stats()
+        ids = []
+
+        def producer():
+            t_next = time.perf_counter()
+            for i in range(n_req):
+                t_next += gaps[i]
+                d = t_next - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                ids.append(srv.submit(*payload[i]))
+
+        th = threading.Thread(target=producer)
+        th.start()
+        th.join()
+        for i in ids:
+            srv.wait(i, 300.0)
+        st = srv.stats()
+        for i in ids:
+            srv.fetch(i).close()
+        span = st["last_done_s"] - st["first_submit_s"]
+        out["rates"].append({
+            "load": mult, "requests_per_s": round(rate, 2), "beams": st["beams_done"],
+            "beamlets": st["beamlets_done"], "beamlets_per_s": round(st["beamlets_done"] / span, 1),
+            "beam_latency_ms": {"p50": round(st["p50_ms"], 3), "p99": round(st["p99_ms"], 3)},
+            "batches": st["batches"], "peak_inflight_batches": st["peak_inflight_batches"]})
+        log(f"serving load {mult}: {st['beamlets_done'] / span:.0f} beamlets/s, p50 {st['p50_ms']:.1f} ms, "
+            f"p99 {st['p99_ms']:.1f} ms")
+    srv.close()
+    ctx.close()
+    return out
+
+
+def leg_spmm(fd, synth, torch, np, cfg, k=256, reps=10):
+    """Precision leg: thresholded (1e-3) dose-loading matrix of the slab beam times 256 fluence vectors,
+    FP32 / BF16 / FP16 values; HBM-bound (writing Y dominates)."""
+    ctx = fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel)
+    ctx.set_threshold(1e-3)
+    dm = fd.dose_mat_construction(ctx, cfg.bundles, cfg.flags, cfg.extent)
+    _, rows, nnz = dm.info()
+    ncols = dm.n_cols
+    X = torch.rand((rows, k), dtype=torch.float32, device="cuda", generator=torch.Generator("cuda").manual_seed(4))
+    out = {"rows": rows, "cols": ncols, "nnz": nnz, "k": k, "types": {}}
+    for name, vt, vb in (("f32", fd.FD_VAL_F32, 4), ("bf16", fd.FD_VAL_BF16, 2), ("fp16", fd.FD_VAL_FP16, 2)):
+        p = fd.Spmm(dm, vt)
+        Y = p.run(X)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            p.run(X, Y)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        bytes_alg = ncols * k * 4 + nnz * (vb + 4) + (ncols + 1) * 8 + rows * k * 4
+        out["types"][name] = {"ms": round(ms, 4), "GB_per_s": round(bytes_alg / ms / 1e6, 1)}
+        p.close()
+    dm.close()
+    ctx.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +276,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the batched 4pi, serving and SpMM legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the beamlet-dose path runs on a single MI355X in this build "
@@ -186,6 +393,10 @@ def main() -> int:
         "cpu_baseline": None,
     }
     ctx.close()
+    if not args.no_extras:
+        out["batched_4pi"], cap = leg_batched_4pi(fd, synth, torch, np)
+        out["serving"] = leg_serving(fd, synth, torch, np, cap)
+        out["spmm_precision"] = leg_spmm(fd, synth, torch, np, cfg)
     if not args.no_cpu_baseline:
         log(f"cpu baseline: oracle on {args.cpu_sample} beamlets")
         out["cpu_baseline"] = cpu_baseline(cfg, beams, flu, args.cpu_sample)

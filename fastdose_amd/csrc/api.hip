@@ -334,6 +334,33 @@ int fd_ctx_create(int device, const fd_volume* vol, const float* density_host, c
         c->spec_dev.mu[e] = spectrum->mu[e];
     }
     c->cones = build_cones(*kernel, &c->n_fwd);
+    // spectrum table over the radiological depth: up to (volume diagonal) x (max density)
+    std::vector<float> tab;
+    {
+        const size_t nv = (size_t)vol->dim[0] * vol->dim[1] * vol->dim[2];
+        float dmax = 0.0f;
+        for (size_t i = 0; i < nv; i++) dmax = std::max(dmax, density_host[i]);
+        double diag = 0.0;
+        for (int d = 0; d < 3; d++) diag += std::pow((double)vol->dim[d] * vol->voxel_size[d], 2);
+        const double rmax = std::sqrt(diag) * std::max((double)dmax, 1e-3) * 1.05 + 1.0;
+        const int ntab = (int)std::min<double>(std::max<double>(std::ceil(rmax / kSpecTableStep) + 2, 256), 1 << 22);
+        const double h = rmax / (ntab - 2);
+        std::vector<double> f(ntab);
+        for (int i = 0; i < ntab; i++) {
+            double acc = 0.0;
+            for (int e = 0; e < spectrum->n; e++)
+                acc += (double)c->spec_dev.w[e] * std::exp(-(double)c->spec_dev.mu[e] * (i * h));
+            f[i] = acc;
+        }
+        tab.resize(2 * (size_t)ntab);
+        for (int i = 0; i < ntab; i++) {
+            const double nxt = i + 1 < ntab ? f[i + 1] : f[i];
+            tab[2 * i] = (float)f[i];
+            tab[2 * i + 1] = (float)(nxt - f[i]);
+        }
+        c->spec_dev.tab_n = ntab;
+        c->spec_dev.tab_inv_h = (float)(1.0 / h);
+    }
     const size_t nvox = (size_t)vol->dim[0] * vol->dim[1] * vol->dim[2];
     auto fail = [&](hipError_t e, const char* what) {
         set_error(std::string(what) + ": " + hipGetErrorString(e));
@@ -344,6 +371,11 @@ int fd_ctx_create(int device, const fd_volume* vol, const float* density_host, c
     if ((e = hipMalloc((void**)&c->density, nvox * sizeof(float))) != hipSuccess) return fail(e, "hipMalloc density");
     if ((e = hipMemcpy(c->density, density_host, nvox * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "density upload");
+    if ((e = hipMalloc((void**)&c->spec_tab, tab.size() * sizeof(float))) != hipSuccess)
+        return fail(e, "hipMalloc spectrum table");
+    if ((e = hipMemcpy(c->spec_tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(e, "spectrum table upload");
+    c->spec_dev.tab = c->spec_tab;
     if ((e = hipMalloc((void**)&c->cones_dev, sizeof(ConeTable))) != hipSuccess) return fail(e, "hipMalloc cones");
     if ((e = hipMemcpy(c->cones_dev, &c->cones, sizeof(ConeTable), hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "cone upload");
@@ -364,6 +396,7 @@ int fd_ctx_destroy(fd_ctx* ctx) {
     (void)hipDeviceSynchronize();
     if (ctx->density) (void)hipFree(ctx->density);
     if (ctx->cones_dev) (void)hipFree(ctx->cones_dev);
+    if (ctx->spec_tab) (void)hipFree(ctx->spec_tab);
     ctx->ws.release();
     ctx->flag_desc.release();
     ctx->flag_out.release();

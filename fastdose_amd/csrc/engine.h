@@ -135,6 +135,7 @@ struct fd_ctx {
     fd_spectrum spectrum{};
     fd_kernel kernel{};
     fdi::SpectrumDev spec_dev{};
+    float* spec_tab = nullptr;  // device spectrum table (spec_dev.tab), (f, df) pairs
     fdi::ConeTable cones{};
     fdi::ConeTable* cones_dev = nullptr;
     int n_fwd = 0;

@@ -63,11 +63,18 @@ struct FlagBeamDev {
 };
 
 // Spectrum folded for the Terma loop: w_e = fluence_e * E_e * mu_e and mu_e.
+// The Terma kernel evaluates sum_e w_e exp(-mu_e r) from a table over the radiological depth r:
+// entry i holds (f(r_i), f(r_{i+1}) - f(r_i)) at r_i = i h, linear in between (h is chosen so the
+// interpolation error stays below ~5e-6 of f).
 struct SpectrumDev {
     int32_t n;
     float w[FD_MAX_TABLE];
     float mu[FD_MAX_TABLE];
+    int32_t tab_n;      // table entries
+    float tab_inv_h;    // 1 / h
+    const float* tab;   // device table, (f, df) pairs
 };
+constexpr double kSpecTableStep = 0.02;  // h <= 0.02 g/cm^2
 
 // Forward cones of the kernel table, heavy (widest) cone first.
 struct ConeTable {

@@ -40,28 +40,82 @@ constexpr int kSlots = kRaysPerLane;  // voxels (rows) per lane
 
 // Distance to the next voxel boundary along one axis, in units of the direction's parameter
 // (d_nextPoint, DoseComp.cu:76-111): ceil(c + 1e-3) - c going up, floor(c - 1e-3) - c going
-// down, infinite for a zero direction component.  Branch-free.
-__device__ __forceinline__ float axis_step(float c, float d, float inv) {
-    const float up = (ceilf(c + kLargerEps) - c) * inv;
-    const float dn = (floorf(c - kLargerEps) - c) * inv;
-    const float st = d > 0.0f ? up : dn;
-    return d == 0.0f ? INFINITY : st;
+// down, infinite for a zero direction component.  With sgn = +1 / -1 for d > 0 / otherwise this
+// is (sgn ceil(sgn c + 1e-3) - c) / d: sgn c + 1e-3 rounds to -(c - 1e-3) exactly when sgn = -1
+// (round-to-nearest is symmetric), so each operation rounds exactly as the two-branch form; for
+// d = +-0 the product is +inf.
+__device__ __forceinline__ float axis_step(float c, float sgn, float inv) {
+    return fmaf(sgn, ceilf(fmaf(sgn, c, kLargerEps)), -c) * inv;
+}
+
+// A wave-uniform float kept in a scalar register.
+__device__ __forceinline__ float uniform(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
 // value of the neighbouring lane in the same 16-lane row (DPP row rotation)
 __device__ __forceinline__ float row_from_left(float v) {   // dst[i] = src[i - 1]
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x121, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x121, 0xF, 0xF, false));
 }
 __device__ __forceinline__ float row_from_right(float v) {  // dst[i] = src[i + 1]
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x12F, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x12F, 0xF, 0xF, false));
 }
 // value of lane src_lane
 __device__ __forceinline__ float wave_from(float v, int src_lane) {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(src_lane << 2, __float_as_int(v)));
 }
 
+// X state follows the shared midpoint voxel: new[x] = old[x - 1] (dx = +1) and so on.
+template <int N>
+__device__ __forceinline__ void shift_x(float (&XA)[N], float (&XB)[N], int d) {
+    for (; d > 0; d--)
+#pragma unroll
+        for (int r = 0; r < N; r++) {
+            XA[r] = row_from_left(XA[r]);
+            XB[r] = row_from_left(XB[r]);
+        }
+    for (; d < 0; d++)
+#pragma unroll
+        for (int r = 0; r < N; r++) {
+            XA[r] = row_from_right(XA[r]);
+            XB[r] = row_from_right(XB[r]);
+        }
+}
+template <int N>
+__device__ __forceinline__ void shift_y(float (&XA)[N], float (&XB)[N], int d, int up_lane, int down_lane) {
+    for (; d > 0; d--) {  // new[y] = old[y - 1]
+        const float ta = wave_from(XA[N - 1], up_lane);
+        const float tb = wave_from(XB[N - 1], up_lane);
+#pragma unroll
+        for (int r = N - 1; r > 0; r--) {
+            XA[r] = XA[r - 1];
+            XB[r] = XB[r - 1];
+        }
+        XA[0] = ta;
+        XB[0] = tb;
+    }
+    for (; d < 0; d++) {  // new[y] = old[y + 1]
+        const float ta = wave_from(XA[0], down_lane);
+        const float tb = wave_from(XB[0], down_lane);
+#pragma unroll
+        for (int r = 0; r < N - 1; r++) {
+            XA[r] = XA[r + 1];
+            XB[r] = XB[r + 1];
+        }
+        XA[N - 1] = ta;
+        XB[N - 1] = tb;
+    }
+}
+
+// Lattice coordinate (within the 16-wide period) of the one ray that crosses the lateral edge in
+// a step from integer cell fb to fn (|fn - fb| <= 1): the ray with offset i crosses iff
+// floor((fb + i) / 16) != floor((fn + i) / 16), i.e. i = -max(fb, fn) mod 16.  -1: none.
+__device__ __forceinline__ int crossing_offset(int fb, int fn) {
+    return fb == fn ? -1 : (-(fb > fn ? fb : fn)) & 15;
+}
+
 template <bool EXACT_G>
-__global__ __launch_bounds__(1024) void cccs_kernel(const BeamletDev* __restrict__ beams, int n,
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8))) void cccs_kernel(const BeamletDev* __restrict__ beams, int n,
                                                     const ConeTable* __restrict__ cones, int n_fwd,
                                                     const float* __restrict__ terma,
                                                     const float* __restrict__ dens_bev,
@@ -83,19 +137,20 @@ __global__ __launch_bounds__(1024) void cccs_kernel(const BeamletDev* __restrict
     const int L = B.L;
 
     // cone constants
-    const float A_a = cones->A_a[t], B_b = cones->B_b[t];
-    const float na = -cones->a[t] * kLog2e, nb = -cones->b[t] * kLog2e;
-    const float ca = cones->a[t], cb = cones->b[t];
+    const float A_a = uniform(cones->A_a[t]), B_b = uniform(cones->B_b[t]);
+    const float na = uniform(-cones->a[t] * kLog2e), nb = uniform(-cones->b[t] * kLog2e);
+    const float ca = uniform(cones->a[t]), cb = uniform(cones->b[t]);
 
     // grid direction of (theta, phi_wave): unit vector scaled to voxel units, renormalised
     const float ux = cones->ux[t][wave], uy = cones->uy[t][wave], uz = cones->uz[t][wave];
     const float vxd = ux / B.bs[0], vyd = uy / B.bs[1], vzd = uz / B.ls;
     const float rn = 1.0f / sqrtf(vxd * vxd + vyd * vyd + vzd * vzd);
-    const float dx = vxd * rn, dy = vyd * rn, dz = vzd * rn;
-    const float ix_ = 1.0f / dx, iy_ = 1.0f / dy, iz_ = 1.0f / dz;
+    const float dx = uniform(vxd * rn), dy = uniform(vyd * rn), dz = uniform(vzd * rn);
+    const float ix_ = uniform(1.0f / dx), iy_ = uniform(1.0f / dy), iz_ = uniform(1.0f / dz);
+    const float sgx = dx > 0.0f ? 1.0f : -1.0f, sgy = dy > 0.0f ? 1.0f : -1.0f, sgz = dz > 0.0f ? 1.0f : -1.0f;
 
-    const float inv_sad = 1.0f / B.sad;
-    const float bsx = B.bs[0], bsy = B.bs[1], ls = B.ls, lim_min = B.lim_min;
+    const float inv_sad = uniform(1.0f / B.sad);
+    const float bsx = uniform(B.bs[0]), bsy = uniform(B.bs[1]), ls = uniform(B.ls), lim_min = uniform(B.lim_min);
 
     // my voxels: x = vx, y = 4 g + r; flat index vbase + 16 r
     const int vx = lane & 15;
@@ -112,6 +167,7 @@ __global__ __launch_bounds__(1024) void cccs_kernel(const BeamletDev* __restrict
     }
     float bxp = 0.5f, byp = 0.5f, bzp = 0.0f;  // march position of this azimuth
     int mx_prev = 0, my_prev = 0;              // midpoint voxel the X state is aligned to
+    int fbx = 0, fby = 0;                      // integer cells of the march position
 
     const float* __restrict__ Tg = terma + B.bev_off;
     const float* __restrict__ Rg = dens_bev + B.bev_off;
@@ -150,9 +206,9 @@ __global__ __launch_bounds__(1024) void cccs_kernel(const BeamletDev* __restrict
 
         for (int it = 0; it < kMaxStepsPerSlice; it++) {
             // next lattice point (d_nextPoint, DoseComp.cu:76-111)
-            const float sx = axis_step(bxp, dx, ix_);
-            const float sy = axis_step(byp, dy, iy_);
-            const float sz = axis_step(bzp, dz, iz_);
+            const float sx = axis_step(bxp, sgx, ix_);
+            const float sy = axis_step(byp, sgy, iy_);
+            const float sz = axis_step(bzp, sgz, iz_);
             const float st = fminf(fminf(sx, sy), sz);
             float nx = fmaf(st, dx, bxp), ny = fmaf(st, dy, byp), nz = fmaf(st, dz, bzp);
             // rint == round here: a tie (.5) is never within 1e-4 of an integer, and the snap is
@@ -164,47 +220,29 @@ __global__ __launch_bounds__(1024) void cccs_kernel(const BeamletDev* __restrict
                 nz = rintf(fmaf(sz, dz, bzp));
             }
 
-            // shared midpoint voxel; rotate the X state to follow it
-            const int mx = (int)floorf((bxp + nx) * 0.5f);
-            const int my = (int)floorf((byp + ny) * 0.5f);
-            int ddx = ((mx - mx_prev + 8) & 15) - 8;
-            int ddy = ((my - my_prev + 8) & 15) - 8;
+            // shared midpoint voxel (wave-uniform); rotate the X state to follow it
+            const int mx = __builtin_amdgcn_readfirstlane((int)floorf((bxp + nx) * 0.5f));
+            const int my = __builtin_amdgcn_readfirstlane((int)floorf((byp + ny) * 0.5f));
+            const int ddx = ((mx - mx_prev + 8) & 15) - 8;
+            const int ddy = ((my - my_prev + 8) & 15) - 8;
             mx_prev = mx;
             my_prev = my;
-            for (; ddx > 0; ddx--)
+            if (ddx == 1) {
 #pragma unroll
                 for (int r = 0; r < kSlots; r++) {
                     XA[r] = row_from_left(XA[r]);
                     XB[r] = row_from_left(XB[r]);
                 }
-            for (; ddx < 0; ddx++)
+            } else if (ddx == -1) {
 #pragma unroll
                 for (int r = 0; r < kSlots; r++) {
                     XA[r] = row_from_right(XA[r]);
                     XB[r] = row_from_right(XB[r]);
                 }
-            for (; ddy > 0; ddy--) {  // new[y] = old[y - 1]
-                const float ta = wave_from(XA[kSlots - 1], up_lane);
-                const float tb = wave_from(XB[kSlots - 1], up_lane);
-#pragma unroll
-                for (int r = kSlots - 1; r > 0; r--) {
-                    XA[r] = XA[r - 1];
-                    XB[r] = XB[r - 1];
-                }
-                XA[0] = ta;
-                XB[0] = tb;
+            } else if (ddx != 0) {
+                shift_x(XA, XB, ddx);
             }
-            for (; ddy < 0; ddy++) {  // new[y] = old[y + 1]
-                const float ta = wave_from(XA[0], down_lane);
-                const float tb = wave_from(XB[0], down_lane);
-#pragma unroll
-                for (int r = 0; r < kSlots - 1; r++) {
-                    XA[r] = XA[r + 1];
-                    XB[r] = XB[r + 1];
-                }
-                XA[kSlots - 1] = ta;
-                XB[kSlots - 1] = tb;
-            }
+            if (ddy != 0) shift_y(XA, XB, ddy, up_lane, down_lane);
 
             // segment length terms shared by all rays: physical lateral offset of a ray at lattice
             // coordinate c is (c - 8) bs f(z), f(z) = (z ls + lim_min) / sad (unwrapped)
@@ -215,24 +253,23 @@ __global__ __launch_bounds__(1024) void cccs_kernel(const BeamletDev* __restrict
             const float axc = fmaf(gx, bxp - 8.0f, bsx * (nx - bxp) * fe);
             const float ayc = fmaf(gy, byp - 8.0f, bsy * (ny - byp) * fe);
             const float dz2 = dzs * dzs;
-            const int fbx = (int)floorf(bxp), fnx = (int)floorf(nx);
-            const int fby = (int)floorf(byp), fny = (int)floorf(ny);
+            const int fnx = __builtin_amdgcn_readfirstlane((int)floorf(nx));
+            const int fny = __builtin_amdgcn_readfirstlane((int)floorf(ny));
 
-            // ray in my column: lane offset, edge crossing, lateral length term
+            // ray in my column: lane offset, lateral length term
             const int ix = (vx - mx) & 15;
-            const bool cross_x = ((fbx + ix) >> 4) != ((fnx + ix) >> 4);
             const float ddxl = fmaf(gx, (float)ix, axc);
             const float dxz2 = fmaf(ddxl, ddxl, dz2);
 #pragma unroll
             for (int r = 0; r < kSlots; r++) {
                 const int iy = (4 * g + r - my) & 15;
-                const bool cross = cross_x || (((fby + iy) >> 4) != ((fny + iy) >> 4));
                 const float ddyl = fmaf(gy, (float)iy, ayc);
                 const float p = __builtin_amdgcn_sqrtf(fmaf(ddyl, ddyl, dxz2)) * R[r];
-                // 1 - e^{-ap}, 1 - e^{-bp} (exactly 0 at p = 0)
-                const float oma = 1.0f - __builtin_amdgcn_exp2f(na * p);
-                const float omb = 1.0f - __builtin_amdgcn_exp2f(nb * p);
+                // e^{-ap}, e^{-bp}
+                const float ea = __builtin_amdgcn_exp2f(na * p);
+                const float eb = __builtin_amdgcn_exp2f(nb * p);
                 if (EXACT_G) {
+                    const float oma = 1.0f - ea, omb = 1.0f - eb;
                     const float ap = ca * p, bp = cb * p;
                     const float ga = ap > 0.0f ? oma * __builtin_amdgcn_rcpf(ap) : 1.0f;
                     const float gb = bp > 0.0f ? omb * __builtin_amdgcn_rcpf(bp) : 1.0f;
@@ -242,14 +279,50 @@ __global__ __launch_bounds__(1024) void cccs_kernel(const BeamletDev* __restrict
                     D[r] = fmaf(fmaf(A_a, XA[r], B_b * XB[r]), p, D[r]);
                 }
                 Cp[r] += p;
-                const float xa = fmaf(oma, T[r] - XA[r], XA[r]);
-                const float xb = fmaf(omb, T[r] - XB[r], XB[r]);
-                XA[r] = cross ? 0.0f : xa;
-                XB[r] = cross ? 0.0f : xb;
+                // X <- T (1 - e) + e X = T - e (T - X)
+                XA[r] = fmaf(-ea, T[r] - XA[r], T[r]);
+                XB[r] = fmaf(-eb, T[r] - XB[r], T[r]);
+            }
+
+            // rays that crossed the lateral BEV edge restart with X = 0 (DoseComp.cu:372-377).  A step
+            // that moves one cell crosses the edge with one ray column and one ray row at most.
+            if (abs(fnx - fbx) <= 1 && abs(fny - fby) <= 1) {
+                const int cxo = crossing_offset(fbx, fnx), cyo = crossing_offset(fby, fny);
+                if (cxo >= 0) {
+                    const bool m = vx == ((mx + cxo) & 15);
+#pragma unroll
+                    for (int r = 0; r < kSlots; r++) {
+                        XA[r] = m ? 0.0f : XA[r];
+                        XB[r] = m ? 0.0f : XB[r];
+                    }
+                }
+                if (cyo >= 0) {
+                    const int yrow = (my + cyo) & 15;  // voxel row of the crossing rays
+                    const bool m = g == (yrow >> 2);
+                    const int rr = yrow & 3;
+#pragma unroll
+                    for (int r = 0; r < kSlots; r++)
+                        if (r == rr) {
+                            XA[r] = m ? 0.0f : XA[r];
+                            XB[r] = m ? 0.0f : XB[r];
+                        }
+                }
+            } else {
+                // general test, ray by ray
+                const bool cx = ((fbx + ix) >> 4) != ((fnx + ix) >> 4);
+#pragma unroll
+                for (int r = 0; r < kSlots; r++) {
+                    const int iy = (4 * g + r - my) & 15;
+                    const bool c = cx || (((fby + iy) >> 4) != ((fny + iy) >> 4));
+                    XA[r] = c ? 0.0f : XA[r];
+                    XB[r] = c ? 0.0f : XB[r];
+                }
             }
             bxp = nx;
             byp = ny;
             bzp = nz;
+            fbx = fnx;
+            fby = fny;
             if (flag) break;
         }
 

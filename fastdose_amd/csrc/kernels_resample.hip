@@ -171,19 +171,34 @@ __device__ float voxel_value(const BeamletDev& B, const float* __restrict__ dose
         qy[i] = ((float)y + o) * vsy - s1;
         qz[i] = ((float)z + o) * vsz - s2;
     }
+    // Projection u = x kx / y + 8 (w alike) with the oracle's rounding: u = fma(x kx, RN(1 / y), 8).
+    // The hardware reciprocal (1 ulp) gives u within a few 1e-6 of that value, so its floor and
+    // range tests agree unless u lies within kNearInt of an integer; only those samples take the
+    // IEEE division.  The rotation's partial products are hoisted out of the sample loops (same
+    // expression, same rounding).
+    constexpr float kNearInt = 4e-5f;
     float acc = 0.0f;
-    for (int kk = 0; kk < kResampleSS; kk++) {
+    for (int ii = 0; ii < kResampleSS; ii++) {
+        const float px = M[0] * qx[ii], py = M[3] * qx[ii], pz = M[6] * qx[ii];
         for (int jj = 0; jj < kResampleSS; jj++) {
+            const float ax = fmaf(M[1], qy[jj], px), ay = fmaf(M[4], qy[jj], py), az = fmaf(M[7], qy[jj], pz);
 #pragma unroll
-            for (int ii = 0; ii < kResampleSS; ii++) {
-                const float bx = fmaf(M[2], qz[kk], fmaf(M[1], qy[jj], M[0] * qx[ii]));
-                const float by = fmaf(M[5], qz[kk], fmaf(M[4], qy[jj], M[3] * qx[ii]));
-                const float bz = fmaf(M[8], qz[kk], fmaf(M[7], qy[jj], M[6] * qx[ii]));
+            for (int kk = 0; kk < kResampleSS; kk++) {
+                const float by = fmaf(M[5], qz[kk], ay);
                 const float dd = (by - B.lim_min) * B.inv_ls;
                 if (!(dd >= 0.0f && dd < Lf)) continue;
-                const float rinv = 1.0f / by;
-                const float u = fmaf(bx * B.kx, rinv, half);
-                const float w = fmaf(bz * B.kz, rinv, half);
+                const float bx = fmaf(M[2], qz[kk], ax);
+                const float bz = fmaf(M[8], qz[kk], az);
+                const float tx = bx * B.kx, tz = bz * B.kz;
+                const float ra = __builtin_amdgcn_rcpf(by);
+                float u = fmaf(tx, ra, half);
+                float w = fmaf(tz, ra, half);
+                const float fu = __builtin_amdgcn_fractf(u), fw = __builtin_amdgcn_fractf(w);
+                if (fu < kNearInt || fu > 1.0f - kNearInt || fw < kNearInt || fw > 1.0f - kNearInt) {
+                    const float rinv = 1.0f / by;
+                    u = fmaf(tx, rinv, half);
+                    w = fmaf(tz, rinv, half);
+                }
                 if (!(u >= 0.0f && u < (float)kFmap && w >= 0.0f && w < (float)kFmap)) continue;
                 acc += D[(size_t)(int)dd * kNpix + (int)w * kFmap + (int)u];
             }
