@@ -39,6 +39,18 @@ def percentile(xs, q):
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
 
 
+def profiled_traffic(name: str):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter summary (FETCH_SIZE +
+    WRITE_SIZE, scaled by the same counters on a copy of known size), or None."""
+    import re
+
+    path = ROOT / "profiles" / name
+    if not path.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", path.read_text())
+    return int(float(m.group(1))) if m else None
+
+
 def cpu_baseline(cfg, beams, flu, sample: int):
     import numpy as np
 
@@ -277,6 +289,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched 4pi, serving and SpMM legs")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one 256 MiB device copy first (reference for HBM traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the beamlet-dose path runs on a single MI355X in this build "
@@ -300,6 +314,12 @@ def main() -> int:
     stride = Lmax * 256
     log(f"workload: {n} beamlets, long_dim {min(b.long_dim for b in beams)}..{Lmax}")
 
+    if args.calib_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     ctx = fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel, options=fd.FD_OPT_TIMING)
     flu_d = torch.from_numpy(flu).to(dev)
     T = torch.zeros(n * stride, dtype=torch.float32, device=dev)
@@ -387,7 +407,7 @@ def main() -> int:
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-            "traffic": None,
+            "traffic": profiled_traffic("cccs_kernel_slab128_traffic.md"),
             "algorithmic": f"{FLOPS_PER_SEGMENT} flop x {nseg} (ray, segment) steps per launch",
         },
         "cpu_baseline": None,

@@ -323,8 +323,11 @@ int fd_ctx_create(int device, const fd_volume* vol, const float* density_host, c
         return FD_ERR_ARG;
     }
     FD_HIP(hipSetDevice(device));
+    int n_cu = 0;
+    FD_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device));
     fd_ctx* c = new fd_ctx();
     c->device = device;
+    c->n_cu = std::max(n_cu, 1);
     c->vol = *vol;
     c->spectrum = *spectrum;
     c->kernel = *kernel;
@@ -409,7 +412,7 @@ int fd_ctx_set_options(fd_ctx* ctx, int options) {
         set_error("null context");
         return FD_ERR_ARG;
     }
-    if (options & ~(FD_OPT_EXACT_G | FD_OPT_RPL_FP64 | FD_OPT_TIMING | FD_OPT_RPL_FP64_ALL)) {
+    if (options & ~(FD_OPT_EXACT_G | FD_OPT_RPL_FP64 | FD_OPT_TIMING | FD_OPT_RPL_FP64_ALL | FD_OPT_GRAPH)) {
         set_error("unknown option bits");
         return FD_ERR_ARG;
     }
@@ -556,7 +559,7 @@ int fd_dense_to_csr(const float* dense_dev, uint64_t rows, uint64_t cols, fd_csr
     } else if (rows > 0) {
         if ((e = launch_fill_u32(cnt, rows, 0, s)) != hipSuccess) return hip_fail(e, "fill");
     }
-    if ((e = launch_scan_u32(cnt, rows, off, off + rows, s)) != hipSuccess) return hip_fail(e, "scan");
+    if ((e = launch_scan_u32(cnt, rows, off, off + rows, nullptr, s)) != hipSuccess) return hip_fail(e, "scan");
     if ((e = hipMemcpyAsync(host, off + rows, sizeof(uint64_t), hipMemcpyDeviceToHost, s)) != hipSuccess)
         return hip_fail(e, "copy");
     if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "sync");

@@ -98,6 +98,7 @@ struct Workspace {
     DevBuf<float> out_vals;
     DevBuf<uint32_t> flags;       // [0]: resampler staging overflow
     DevBuf<uint32_t> rowmax;      // per-beamlet maximum (float bits) for the relative threshold
+    DevBuf<uint64_t> scan_tmp;    // tile sums of the multi-workgroup scans
     uint64_t* host_totals = nullptr;  // pinned: candidates, nonzeros, overflow flag
     uint64_t* host_rowoff = nullptr;  // pinned copy of rowoff
     size_t host_rowoff_cap = 0;
@@ -106,6 +107,8 @@ struct Workspace {
     hipEvent_t ev[2 * FD_T_COUNT] = {};
     bool ev_used[FD_T_COUNT] = {};
     bool timing = false;
+    hipGraphExec_t gexec = nullptr;   // last instantiated batch graph (FD_OPT_GRAPH)
+    hipStream_t own_stream = nullptr; // capture stream when the caller passes the null stream
 
     int init();
     void release();
@@ -141,6 +144,7 @@ struct fd_ctx {
     int n_fwd = 0;
     int options = 0;
     float threshold = 0.0f;  // relative CSR threshold (0: drop exact zeros only)
+    int n_cu = 256;          // compute units of the device
     fdi::Workspace ws;
     fdi::DevBuf<fdi::FlagBeamDev> flag_desc;  // beamlet-flag kernel inputs / outputs
     fdi::DevBuf<uint8_t> flag_out;

@@ -21,6 +21,10 @@ int Workspace::init() {
 }
 
 void Workspace::release() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+    if (own_stream) (void)hipStreamDestroy(own_stream);
+    own_stream = nullptr;
     if (stage) (void)hipHostFree(stage);
     if (flu_stage) (void)hipHostFree(flu_stage);
     if (host_totals) (void)hipHostFree(host_totals);
@@ -52,6 +56,7 @@ void Workspace::release() {
     out_vals.release();
     flags.release();
     rowmax.release();
+    scan_tmp.release();
 }
 
 size_t Workspace::footprint() const {
@@ -172,9 +177,14 @@ int run_terma(const fd_ctx* ctx, Workspace& w, int options, int n, const float* 
     return FD_OK;
 }
 
+int reserve_cccs(const fd_ctx* ctx, Workspace& w, int n, uint64_t stride) {
+    FD_TRY(w.partial.reserve((size_t)n * std::max(ctx->n_fwd, 1) * stride));
+    return FD_OK;
+}
+
 int run_cccs(const fd_ctx* ctx, Workspace& w, int options, int n, int max_L, const float* terma, const float* dens,
              float* dose, uint64_t stride, hipStream_t stream) {
-    FD_TRY(w.partial.reserve((size_t)n * std::max(ctx->n_fwd, 1) * stride));
+    FD_TRY(reserve_cccs(ctx, w, n, stride));
     hipEvent_t evs[3];
     const bool t = w.timing && w.ev[0];
     if (t) {
@@ -205,8 +215,9 @@ int reserve_segments(const fd_ctx* ctx, Workspace& w, int n) {
     FD_TRY(w.nnz_count.reserve(segs));
     FD_TRY(w.cand_off.reserve(segs + 1));
     FD_TRY(w.nnz_off.reserve(segs + 1));
-    FD_TRY(w.flags.reserve(1));
+    FD_TRY(w.flags.reserve(2));  // [0] overflow, [1] zero (read as one u64 with [0])
     FD_TRY(w.rowmax.reserve(n));
+    FD_TRY(w.scan_tmp.reserve(scan_scratch_size(segs)));
     return FD_OK;
 }
 
@@ -214,7 +225,7 @@ int launch_count(const fd_ctx* ctx, Workspace& w, int n, hipStream_t stream) {
     const size_t segs = (size_t)n * ctx->vol.dim[2];
     w.mark(FD_T_RESAMPLE_COUNT, 0, stream);
     hipError_t e = launch_resample_count(w.desc.ptr, n, ctx->vol, w.cand_count.ptr, stream);
-    if (e == hipSuccess) e = launch_scan_u32(w.cand_count.ptr, segs, w.cand_off.ptr, w.cand_off.ptr + segs, stream);
+    if (e == hipSuccess) e = launch_scan_u32(w.cand_count.ptr, segs, w.cand_off.ptr, w.cand_off.ptr + segs, w.scan_tmp.ptr, stream);
     w.mark(FD_T_RESAMPLE_COUNT, 1, stream);
     if (e != hipSuccess) {
         set_error(std::string("resample count: ") + hipGetErrorString(e));
@@ -228,7 +239,7 @@ int launch_values(const fd_ctx* ctx, Workspace& w, int n, const float* dose, flo
     const size_t segs = (size_t)n * ctx->vol.dim[2];
     const int Z = (int)ctx->vol.dim[2];
     w.mark(FD_T_RESAMPLE_VALUES, 0, stream);
-    hipError_t e = launch_fill_u32(w.flags.ptr, 1, 0u, stream);
+    hipError_t e = launch_fill_u32(w.flags.ptr, 2, 0u, stream);
     if (e == hipSuccess && thr > 0.0f) e = launch_fill_u32(w.rowmax.ptr, (uint64_t)n, 0u, stream);
     if (e == hipSuccess)
         e = launch_resample_values(w.desc.ptr, n, ctx->vol, extent, dose, w.cand_off.ptr, cap, w.flags.ptr,
@@ -240,9 +251,9 @@ int launch_values(const fd_ctx* ctx, Workspace& w, int n, const float* dose, flo
             e = launch_resample_threshold_count(n, Z, w.cand_off.ptr, w.nnz_count.ptr, w.tmp_vals.ptr, w.rowmax.ptr,
                                                 thr, w.cand_count.ptr, stream);
         if (e == hipSuccess)
-            e = launch_scan_u32(w.cand_count.ptr, segs, w.nnz_off.ptr, w.nnz_off.ptr + segs, stream);
+            e = launch_scan_u32(w.cand_count.ptr, segs, w.nnz_off.ptr, w.nnz_off.ptr + segs, w.scan_tmp.ptr, stream);
     } else if (e == hipSuccess) {
-        e = launch_scan_u32(w.nnz_count.ptr, segs, w.nnz_off.ptr, w.nnz_off.ptr + segs, stream);
+        e = launch_scan_u32(w.nnz_count.ptr, segs, w.nnz_off.ptr, w.nnz_off.ptr + segs, w.scan_tmp.ptr, stream);
     }
     w.mark(FD_T_RESAMPLE_VALUES, 1, stream);
     if (e != hipSuccess) {
@@ -252,26 +263,26 @@ int launch_values(const fd_ctx* ctx, Workspace& w, int n, const float* dose, flo
     return FD_OK;
 }
 
+int reserve_host_rowoff(Workspace& w, int n) {
+    if ((size_t)n + 1 <= w.host_rowoff_cap) return FD_OK;
+    if (w.host_rowoff) FD_HIP(hipHostFree(w.host_rowoff));
+    w.host_rowoff = nullptr;
+    w.host_rowoff_cap = 0;
+    FD_HIP(hipHostMalloc((void**)&w.host_rowoff, ((size_t)n + 1) * sizeof(uint64_t), hipHostMallocDefault));
+    w.host_rowoff_cap = (size_t)n + 1;
+    return FD_OK;
+}
+
 // totals of the last resampler pass -> pinned host: [0] candidates, [1] nonzeros, [2] overflow;
-// with rows, also the batch's row offsets -> w.host_rowoff
+// with rows, also the batch's row offsets -> w.host_rowoff (reserved by the caller)
 int fetch_totals(const fd_ctx* ctx, Workspace& w, int n, hipStream_t stream, bool rows) {
     const size_t segs = (size_t)n * ctx->vol.dim[2];
-    w.host_totals[2] = 0;
-    if (rows) {
-        if ((size_t)n + 1 > w.host_rowoff_cap) {
-            if (w.host_rowoff) FD_HIP(hipHostFree(w.host_rowoff));
-            w.host_rowoff = nullptr;
-            w.host_rowoff_cap = 0;
-            const size_t want = std::max<size_t>((size_t)n + 1, 2 * w.host_rowoff_cap);
-            FD_HIP(hipHostMalloc((void**)&w.host_rowoff, want * sizeof(uint64_t), hipHostMallocDefault));
-            w.host_rowoff_cap = want;
-        }
+    if (rows)
         FD_HIP(hipMemcpyAsync(w.host_rowoff, w.rowoff.ptr, ((size_t)n + 1) * sizeof(uint64_t),
                               hipMemcpyDeviceToHost, stream));
-    }
     FD_HIP(hipMemcpyAsync(w.host_totals, w.cand_off.ptr + segs, sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
     FD_HIP(hipMemcpyAsync(w.host_totals + 1, w.nnz_off.ptr + segs, sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
-    FD_HIP(hipMemcpyAsync(w.host_totals + 2, w.flags.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    FD_HIP(hipMemcpyAsync(w.host_totals + 2, w.flags.ptr, sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
     return FD_OK;
 }
 
@@ -333,17 +344,60 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
     FD_TRY(w.out_cols.reserve(cap));
     FD_TRY(w.out_vals.reserve(cap));
 
+    FD_TRY(reserve_cccs(ctx, w, n, stride));
+    FD_TRY(reserve_host_rowoff(w, n));
     FD_TRY(stage_inputs(w, beams, n, stride, fluence));
-    FD_HIP(hipMemcpyAsync(w.desc.ptr, w.stage, (size_t)n * sizeof(BeamletDev), hipMemcpyHostToDevice, stream));
-    FD_HIP(hipMemcpyAsync(w.fluence.ptr, w.flu_stage, (size_t)n * kNpix * sizeof(float), hipMemcpyHostToDevice,
-                          stream));
+
+    // Everything from the uploads to the host copies of the totals is one stream-ordered sequence
+    // with no host synchronisation, so it can run as a captured HIP graph (FD_OPT_GRAPH).
+    auto enqueue = [&](hipStream_t st) -> int {
+        FD_HIP(hipMemcpyAsync(w.desc.ptr, w.stage, (size_t)n * sizeof(BeamletDev), hipMemcpyHostToDevice, st));
+        FD_HIP(hipMemcpyAsync(w.fluence.ptr, w.flu_stage, (size_t)n * kNpix * sizeof(float), hipMemcpyHostToDevice,
+                              st));
+        FD_TRY(run_terma(ctx, w, options, n, w.fluence.ptr, w.terma.ptr, w.dens_bev.ptr, st));
+        FD_TRY(run_cccs(ctx, w, options, n, maxL, w.terma.ptr, w.dens_bev.ptr, w.dose_bev.ptr, stride, st));
+        FD_TRY(launch_count(ctx, w, n, st));
+        FD_TRY(launch_values(ctx, w, n, w.dose_bev.ptr, extent, thr, cap, st));
+        FD_TRY(run_resample_compact(ctx, w, n, thr, w.rowoff.ptr, w.out_cols.ptr, w.out_vals.ptr, st));
+        FD_TRY(fetch_totals(ctx, w, n, st, true));
+        return FD_OK;
+    };
+    if (options & FD_OPT_GRAPH) {
+        if (!stream) {
+            if (!w.own_stream) FD_HIP(hipStreamCreateWithFlags(&w.own_stream, hipStreamNonBlocking));
+            stream = w.own_stream;
+        }
+        const bool timing = w.timing;
+        w.timing = false;  // no event records inside the graph
+        FD_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        const int r = enqueue(stream);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(stream, &g);
+        w.timing = timing;
+        if (r != FD_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return r;
+        }
+        FD_HIP(ec);
+        if (w.gexec) {
+            // same node sequence as the last batch: update its parameters in place
+            hipGraphNode_t bad = nullptr;
+            hipGraphExecUpdateResult res;
+            if (hipGraphExecUpdate(w.gexec, g, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(w.gexec);
+                w.gexec = nullptr;
+            }
+        }
+        hipError_t ei = hipSuccess;
+        if (!w.gexec) ei = hipGraphInstantiate(&w.gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        FD_HIP(ei);
+        FD_HIP(hipGraphLaunch(w.gexec, stream));
+    } else {
+        FD_TRY(enqueue(stream));
+    }
     FD_HIP(hipEventRecord(w.stage_done, stream));
-    FD_TRY(run_terma(ctx, w, options, n, w.fluence.ptr, w.terma.ptr, w.dens_bev.ptr, stream));
-    FD_TRY(run_cccs(ctx, w, options, n, maxL, w.terma.ptr, w.dens_bev.ptr, w.dose_bev.ptr, stride, stream));
-    FD_TRY(launch_count(ctx, w, n, stream));
-    FD_TRY(launch_values(ctx, w, n, w.dose_bev.ptr, extent, thr, cap, stream));
-    FD_TRY(run_resample_compact(ctx, w, n, thr, w.rowoff.ptr, w.out_cols.ptr, w.out_vals.ptr, stream));
-    FD_TRY(fetch_totals(ctx, w, n, stream, true));
     FD_HIP(hipStreamSynchronize(stream));
     const uint64_t ncand = w.host_totals[0];
     const double ratio = est > 0 ? (double)ncand / est : 1.0;

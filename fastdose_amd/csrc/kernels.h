@@ -47,9 +47,11 @@ hipError_t launch_resample_compact(int n, int Z, const uint64_t* cand_off, const
                                    const uint32_t* rowmax_bits, float thr, uint64_t* offsets, uint64_t* columns,
                                    float* values, hipStream_t stream);
 
-// Exclusive scan of count[0..N) (u32) into off[0..N) (u64) plus off_total[0] = sum.
-hipError_t launch_scan_u32(const uint32_t* count, uint64_t N, uint64_t* off, uint64_t* total,
+// Exclusive scan of count[0..N) (u32) into off[0..N) (u64) plus total[0] = sum.  With scratch
+// (scan_scratch_size(N) u64, nullable) large inputs take a three-pass multi-workgroup scan.
+hipError_t launch_scan_u32(const uint32_t* count, uint64_t N, uint64_t* off, uint64_t* total, uint64_t* scratch,
                            hipStream_t stream);
+uint64_t scan_scratch_size(uint64_t N);
 
 // Dense (rows x cols, row-major) -> CSR: per-row nonzero counts, then fill.
 hipError_t launch_dense_count(const float* dense, uint64_t rows, uint64_t cols, uint32_t* row_count,

@@ -42,6 +42,88 @@ __global__ __launch_bounds__(kScanThreads) void scan_u32_kernel(const uint32_t* 
     if (threadIdx.x == kScanThreads - 1) total[0] = s_part[kScanThreads - 1];
 }
 
+// Three-pass scan for large inputs: tile sums, a scan of the tile sums, tile scans.  A tile is
+// 4096 elements, 16 consecutive ones per thread.
+constexpr int kTileThreads = 256;
+constexpr int kTileItems = 16;
+constexpr uint64_t kTile = (uint64_t)kTileThreads * kTileItems;
+
+__device__ __forceinline__ uint64_t block_excl_scan_u64(uint64_t v, uint64_t* s_wave, uint64_t& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    uint64_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wave[wave] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t acc = 0;
+        for (int w = 0; w < nw; w++) {
+            const uint64_t t = s_wave[w];
+            s_wave[w] = acc;
+            acc += t;
+        }
+        s_wave[nw] = acc;
+    }
+    __syncthreads();
+    const uint64_t res = s_wave[wave] + x - v;
+    total = s_wave[nw];
+    __syncthreads();
+    return res;
+}
+
+__global__ __launch_bounds__(kTileThreads) void scan_tile_sums_kernel(const uint32_t* __restrict__ count, uint64_t N,
+                                                                      uint64_t* __restrict__ sums) {
+    __shared__ uint64_t s_wave[kTileThreads / 64 + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kTileItems;
+    uint64_t v = 0;
+#pragma unroll
+    for (int i = 0; i < kTileItems; i++)
+        if (base + i < N) v += count[base + i];
+    uint64_t total;
+    block_excl_scan_u64(v, s_wave, total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// exclusive scan of the tile sums in place (one workgroup), grand total to total[0]
+__global__ __launch_bounds__(1024) void scan_sums_kernel(uint64_t* __restrict__ sums, uint64_t nb,
+                                                         uint64_t* __restrict__ total) {
+    __shared__ uint64_t s_wave[1024 / 64 + 1];
+    uint64_t carry = 0;
+    for (uint64_t i0 = 0; i0 < nb; i0 += blockDim.x) {
+        const uint64_t i = i0 + threadIdx.x;
+        const uint64_t v = i < nb ? sums[i] : 0;
+        uint64_t tot;
+        const uint64_t ex = block_excl_scan_u64(v, s_wave, tot);
+        if (i < nb) sums[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) total[0] = carry;
+}
+
+__global__ __launch_bounds__(kTileThreads) void scan_tiles_kernel(const uint32_t* __restrict__ count, uint64_t N,
+                                                                  const uint64_t* __restrict__ sums,
+                                                                  uint64_t* __restrict__ off) {
+    __shared__ uint64_t s_wave[kTileThreads / 64 + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kTileItems;
+    uint32_t c[kTileItems];
+    uint64_t v = 0;
+#pragma unroll
+    for (int i = 0; i < kTileItems; i++) {
+        c[i] = base + i < N ? count[base + i] : 0u;
+        v += c[i];
+    }
+    uint64_t total;
+    uint64_t run = sums[blockIdx.x] + block_excl_scan_u64(v, s_wave, total);
+#pragma unroll
+    for (int i = 0; i < kTileItems; i++) {
+        if (base + i < N) off[base + i] = run;
+        run += c[i];
+    }
+}
+
 __global__ __launch_bounds__(256) void dense_count_kernel(const float* __restrict__ dense, uint64_t cols,
                                                           uint32_t* __restrict__ row_count) {
     __shared__ uint32_t s_w[4];
@@ -91,9 +173,18 @@ __global__ void write_offsets_kernel(const uint64_t* __restrict__ row_off, const
 
 }  // namespace
 
-hipError_t launch_scan_u32(const uint32_t* count, uint64_t N, uint64_t* off, uint64_t* total,
+uint64_t scan_scratch_size(uint64_t N) { return (N + kTile - 1) / kTile + 1; }
+
+hipError_t launch_scan_u32(const uint32_t* count, uint64_t N, uint64_t* off, uint64_t* total, uint64_t* scratch,
                            hipStream_t stream) {
-    hipLaunchKernelGGL(scan_u32_kernel, dim3(1), dim3(kScanThreads), 0, stream, count, N, off, total);
+    if (!scratch || N <= kTile) {
+        hipLaunchKernelGGL(scan_u32_kernel, dim3(1), dim3(kScanThreads), 0, stream, count, N, off, total);
+        return hipGetLastError();
+    }
+    const uint64_t nb = (N + kTile - 1) / kTile;
+    hipLaunchKernelGGL(scan_tile_sums_kernel, dim3((unsigned)nb), dim3(kTileThreads), 0, stream, count, N, scratch);
+    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, stream, scratch, nb, total);
+    hipLaunchKernelGGL(scan_tiles_kernel, dim3((unsigned)nb), dim3(kTileThreads), 0, stream, count, N, scratch, off);
     return hipGetLastError();
 }
 

@@ -120,7 +120,19 @@ __device__ __forceinline__ float entry_value(const void* cval, uint64_t e, const
     return (VT == FD_VAL_BF16 ? bf16_to_f32(h) : fp16_to_f32(h)) * scale[r];
 }
 
-// Y[c][j] = sum over column c's entries (r, v) of v * X[r][j]; one wavefront per column.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// 64-bit value of lane `src` (wave-uniform result)
+__device__ __forceinline__ uint64_t lane_u64(uint64_t v, int src) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Y[c][j] = sum over column c's entries (r, v) of v * X[r][j].  A wavefront takes tiles of 64
+// consecutive columns: one coalesced load brings the tile's column pointers into lanes, then the
+// wavefront walks the columns (pointers read across lanes), lanes along j (4 per lane, 16-byte
+// X loads), and streams each Y row out with one 16-byte store per lane (empty columns store zeros).
 template <int VT>
 __global__ __launch_bounds__(256) void spmm_kernel(uint64_t C, const uint64_t* __restrict__ colptr,
                                                    const uint32_t* __restrict__ crow, const void* __restrict__ cval,
@@ -129,30 +141,40 @@ __global__ __launch_bounds__(256) void spmm_kernel(uint64_t C, const uint64_t* _
     const int lane = threadIdx.x & 63;
     const bool vec = (k & 3) == 0;  // rows of X and Y 16-byte aligned
     const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    for (uint64_t c = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; c < C; c += waves) {
-        const uint64_t a = colptr[c], b = colptr[c + 1];
-        for (int j0 = 0; j0 < k; j0 += 256) {
-            const int j = j0 + 4 * lane;
-            float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (vec && j + 3 < k) {
-                for (uint64_t e = a; e < b; e++) {
-                    const uint32_t r = crow[e];
-                    const float v = entry_value<VT>(cval, e, scale, r);
-                    const float4 x = *(const float4*)(X + (uint64_t)r * k + j);
-                    acc.x = fmaf(v, x.x, acc.x);
-                    acc.y = fmaf(v, x.y, acc.y);
-                    acc.z = fmaf(v, x.z, acc.z);
-                    acc.w = fmaf(v, x.w, acc.w);
+    const uint64_t tiles = (C + 63) / 64;
+    for (uint64_t tile = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; tile < tiles; tile += waves) {
+        const uint64_t c0 = tile * 64;
+        const uint64_t cl = c0 + lane < C ? c0 + lane : C;
+        const uint64_t lo = colptr[cl];
+        const uint64_t hi = colptr[cl + 1 <= C ? cl + 1 : C];
+        const int ncol = (int)(C - c0 < 64 ? C - c0 : 64);
+        for (int jc = 0; jc < ncol; jc++) {
+            const uint64_t c = c0 + jc;
+            const uint64_t a = lane_u64(lo, jc), e_end = lane_u64(hi, jc);
+            for (int j0 = 0; j0 < k; j0 += 256) {
+                const int j = j0 + 4 * lane;
+                if (vec && j + 3 < k) {
+                    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    for (uint64_t e = a; e < e_end; e++) {
+                        const uint32_t r = crow[e];
+                        const float v = entry_value<VT>(cval, e, scale, r);
+                        const float4 x = *(const float4*)(X + (uint64_t)r * k + j);
+                        acc.x = fmaf(v, x.x, acc.x);
+                        acc.y = fmaf(v, x.y, acc.y);
+                        acc.z = fmaf(v, x.z, acc.z);
+                        acc.w = fmaf(v, x.w, acc.w);
+                    }
+                    const v4f o = {acc.x, acc.y, acc.z, acc.w};
+                    __builtin_nontemporal_store(o, (v4f*)(Y + c * (uint64_t)k + j));
+                } else if (j < k) {
+                    float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                    for (uint64_t e = a; e < e_end; e++) {
+                        const uint32_t r = crow[e];
+                        const float v = entry_value<VT>(cval, e, scale, r);
+                        for (int i = 0; i < 4 && j + i < k; i++) t[i] = fmaf(v, X[(uint64_t)r * k + j + i], t[i]);
+                    }
+                    for (int i = 0; i < 4 && j + i < k; i++) Y[c * (uint64_t)k + j + i] = t[i];
                 }
-                *(float4*)(Y + c * (uint64_t)k + j) = acc;
-            } else if (j < k) {
-                float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-                for (uint64_t e = a; e < b; e++) {
-                    const uint32_t r = crow[e];
-                    const float v = entry_value<VT>(cval, e, scale, r);
-                    for (int i = 0; i < 4 && j + i < k; i++) t[i] = fmaf(v, X[(uint64_t)r * k + j + i], t[i]);
-                }
-                for (int i = 0; i < 4 && j + i < k; i++) Y[c * (uint64_t)k + j + i] = t[i];
             }
         }
     }
@@ -215,7 +237,7 @@ hipError_t launch_col_sort(uint64_t C, const uint64_t* colptr, uint32_t* crow, v
 hipError_t launch_spmm(int vtype, uint64_t C, const uint64_t* colptr, const uint32_t* crow, const void* cval,
                        const float* scale, const float* X, int k, float* Y, hipStream_t stream) {
     if (C == 0 || k <= 0) return hipSuccess;
-    const dim3 g(blocks_for(C * 64, 8192)), b(256);
+    const dim3 g(blocks_for((C + 63) / 64 * 64, 4096)), b(256);
     if (vtype == FD_VAL_F32)
         hipLaunchKernelGGL(spmm_kernel<FD_VAL_F32>, g, b, 0, stream, C, colptr, crow, cval, scale, X, k, Y);
     else if (vtype == FD_VAL_BF16)

@@ -143,13 +143,17 @@ int fd_spmm_create(const fd_dosemat* m, int value_type, fd_spmm** out, void* str
                                     vbytes == 2 ? q.ptr : nullptr, s);
     if (e == hipSuccess) e = launch_fill_u32(count.ptr, p->cols, 0u, s);
     if (e == hipSuccess) e = launch_col_count(p->nnz, m->columns.ptr, count.ptr, s);
-    if (e == hipSuccess) e = launch_scan_u32(count.ptr, p->cols, p->colptr.ptr, p->colptr.ptr + p->cols, s);
+    DevBuf<uint64_t> scan_tmp;
+    if (e == hipSuccess && scan_tmp.reserve(scan_scratch_size(p->cols)) != FD_OK) e = hipErrorOutOfMemory;
+    if (e == hipSuccess)
+        e = launch_scan_u32(count.ptr, p->cols, p->colptr.ptr, p->colptr.ptr + p->cols, scan_tmp.ptr, s);
     if (e == hipSuccess) e = launch_fill_u32(count.ptr, p->cols, 0u, s);
     if (e == hipSuccess)
         e = launch_col_scatter(p->rows, rowptr.ptr, m->columns.ptr, src, vbytes, p->colptr.ptr, count.ptr,
                                p->crow.ptr, p->cval.ptr, s);
     if (e == hipSuccess) e = launch_col_sort(p->cols, p->colptr.ptr, p->crow.ptr, p->cval.ptr, vbytes, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
+    scan_tmp.release();
     if (e != hipSuccess) {
         set_error(std::string("spmm build: ") + hipGetErrorString(e));
         return fail(FD_ERR_HIP);

@@ -47,7 +47,9 @@ enum {
     FD_OPT_RPL_FP64 = 2,     /* accumulate the Terma radiological path in FP64 on rays longer
                                 than 256 voxels (long_dim * long_spacing > 256 * min voxel size) */
     FD_OPT_TIMING = 4,       /* record HIP events around every kernel launch (fd_ctx_timings) */
-    FD_OPT_RPL_FP64_ALL = 8  /* FP64 radiological path on every ray */
+    FD_OPT_RPL_FP64_ALL = 8, /* FP64 radiological path on every ray */
+    FD_OPT_GRAPH = 16        /* batched paths (fd_dosemat_add_beam(s), serving) replay each batch as one
+                                captured HIP graph instead of separate launches */
 };
 
 /* Value types of packed dose-loading matrices (fd_dosemat_pack, fd_spmm_create). */

@@ -63,7 +63,10 @@ def check_against_oracle(cfg, beams, flu, g, idx, flags=0, stage_isolated=True):
             c2, v2, _ = orc.resample(orc.cast(orc.Beam, b), D.astype(np.float64), orc.cast(orc.Volume, cfg.volume),
                                      cfg.extent)
             assert np.array_equal(cols.astype(np.int64), c2), i
-            assert np.allclose(vals, v2, rtol=2e-6, atol=1e-30 + 1e-7 * v2.max())
+            # same samples, same BEV pixels; the GPU sums up to 125 of them in FP32 (relative error
+            # <= 124 * 2^-24 < 1e-5), the oracle in FP64
+            rel = np.abs(vals - v2) / np.maximum(np.abs(v2), 1e-30)
+            assert np.all((rel <= 1e-5) | (np.abs(vals - v2) <= 1e-7 * v2.max())), (i, float(rel.max()))
     log(f"worst rel errors {worst}")
     return worst
 
@@ -258,6 +261,35 @@ def test_rpl_fp64_rule(torch_cuda):
         ref = oracle_beamlet(cfg, beams[i], flu[i])
         et = np.abs(g["terma"][i, :L] - ref["terma"]).max() / ref["terma"].max()
         assert et < 1e-4, (i, et)
+
+
+def test_graph_replays_match_eager(torch_cuda):
+    """FD_OPT_GRAPH: each batch captured into a HIP graph and replayed (instantiated once, then
+    updated in place); every replay equals the eager launches bit for bit, with eager work and
+    synchronisations between the replays."""
+    torch = torch_cuda
+    cfg = synth.small_config(seed=18, dim=(28, 26, 24), fluence_dim=3)
+    iso = cfg.bundles[0].isocenter
+    for ang in ((200.0, 10.0, 5.0), (75.0, -40.0, 0.0)):
+        cfg.bundles.append(synth.make_bundle(iso, ang, fluence_dim=3))
+        cfg.flags.append(np.ones(9, dtype=np.uint8))
+    with fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel) as ctx:
+        ref = fd.dose_mat_construction(ctx, cfg.bundles, cfg.flags, cfg.extent)
+        ctx.set_options(fd.FD_OPT_GRAPH)
+        for rep in range(4):
+            log(f"graph replay {rep}")
+            chunk = (0, 7, 4, 0)[rep]
+            dm = fd.dose_mat_construction(ctx, cfg.bundles, cfg.flags, cfg.extent, batched=True, max_chunk=chunk)
+            x = torch.ones(1 << 20, device="cuda")  # eager work between replays
+            assert float(x.sum()) == float(1 << 20)
+            torch.cuda.synchronize()
+            for k in range(len(cfg.bundles)):
+                assert all(np.array_equal(a, b) for a, b in zip(ref.matrix(k), dm.matrix(k))), (rep, k)
+        with fd.Server(ctx, n_slots=2, max_batch_beamlets=10, options=fd.FD_OPT_GRAPH, extent=cfg.extent) as srv:
+            beams, flu, counts = fd.bundle_beamlets(cfg.bundles, cfg.flags, cfg.volume)
+            dm = srv.fetch(srv.submit(beams, flu, counts))
+        for k in range(len(cfg.bundles)):
+            assert all(np.array_equal(a, b) for a, b in zip(ref.matrix(k), dm.matrix(k))), ("serve", k)
 
 
 def test_error_paths(torch_cuda):
