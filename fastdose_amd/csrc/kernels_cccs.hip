@@ -114,17 +114,17 @@ __device__ __forceinline__ int crossing_offset(int fb, int fn) {
     return fb == fn ? -1 : (-(fb > fn ? fb : fn)) & 15;
 }
 
-template <bool EXACT_G>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8))) void cccs_kernel(const BeamletDev* __restrict__ beams, int n,
-                                                    const ConeTable* __restrict__ cones, int n_fwd,
-                                                    const float* __restrict__ terma,
-                                                    const float* __restrict__ dens_bev,
-                                                    float* __restrict__ partial, uint64_t stride) {
-    extern __shared__ float smem[];
+// Body of the CCCS kernel; CUMU: this wavefront carries the last azimuth and accumulates the
+// cumulative p used for the slice normalisation (the other wavefronts skip it).
+template <bool EXACT_G, bool CUMU>
+__device__ __forceinline__ void cccs_body(const BeamletDev* __restrict__ beams, int n,
+                                          const ConeTable* __restrict__ cones, int n_fwd,
+                                          const float* __restrict__ terma, const float* __restrict__ dens_bev,
+                                          float* __restrict__ partial, uint64_t stride, float* smem,
+                                          int* s_next) {
     float* sTR = smem;             // [2 buffers][T | rho][256]
     float* sC = smem + 4 * kNpix;  // [2 buffers][256]
     float* sD = smem + 6 * kNpix;  // [2 buffers][n_phi][256]
-    __shared__ int s_next[2];
 
     const int t = blockIdx.x / n;  // forward-cone slot, widest cone first
     const int b = blockIdx.x - t * n;
@@ -213,7 +213,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8))) vo
             float nx = fmaf(st, dx, bxp), ny = fmaf(st, dy, byp), nz = fmaf(st, dz, bzp);
             // rint == round here: a tie (.5) is never within 1e-4 of an integer, and the snap is
             // only applied to values within 1e-4 of one
-            const bool flag = fabsf(nz - rintf(nz)) < kEps;
+            // wave-uniform: taken as a scalar branch (one step per slice)
+            const bool flag = __builtin_amdgcn_readfirstlane(fabsf(nz - rintf(nz)) < kEps ? 1 : 0) != 0;
             if (flag) {
                 nx = fmaf(sz, dx, bxp);
                 ny = fmaf(sz, dy, byp);
@@ -278,7 +279,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8))) vo
                 } else {
                     D[r] = fmaf(fmaf(A_a, XA[r], B_b * XB[r]), p, D[r]);
                 }
-                Cp[r] += p;
+                if (CUMU) Cp[r] += p;
                 // X <- T (1 - e) + e X = T - e (T - X)
                 XA[r] = fmaf(-ea, T[r] - XA[r], T[r]);
                 XB[r] = fmaf(-eb, T[r] - XB[r], T[r]);
@@ -365,6 +366,20 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8))) vo
     }
     for (int z = written; z < L; z++)
         for (int i = tid; i < kNpix; i += nthreads) P[(size_t)z * kNpix + i] = 0.0f;
+}
+
+template <bool EXACT_G>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8))) void cccs_kernel(
+    const BeamletDev* __restrict__ beams, int n, const ConeTable* __restrict__ cones, int n_fwd,
+    const float* __restrict__ terma, const float* __restrict__ dens_bev, float* __restrict__ partial,
+    uint64_t stride) {
+    // one LDS allocation shared by both instantiations of the body
+    extern __shared__ float smem[];
+    __shared__ int s_next[2];
+    if ((threadIdx.x >> 6) == (blockDim.x >> 6) - 1)
+        cccs_body<EXACT_G, true>(beams, n, cones, n_fwd, terma, dens_bev, partial, stride, smem, s_next);
+    else
+        cccs_body<EXACT_G, false>(beams, n, cones, n_fwd, terma, dens_bev, partial, stride, smem, s_next);
 }
 
 // dose[b][s][p] = sum over forward cones of partial[b][t][s][p], cone slots in order.
