@@ -369,17 +369,508 @@ __device__ __forceinline__ void cccs_body(const BeamletDev* __restrict__ beams, 
 }
 
 template <bool EXACT_G>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8))) void cccs_kernel(
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8))) void cccs_inline_kernel(
     const BeamletDev* __restrict__ beams, int n, const ConeTable* __restrict__ cones, int n_fwd,
     const float* __restrict__ terma, const float* __restrict__ dens_bev, float* __restrict__ partial,
     uint64_t stride) {
-    // one LDS allocation shared by both instantiations of the body
+    // n_phi = 16: one wavefront per azimuth, each marching itself.  One LDS allocation shared by both instantiations of the body
     extern __shared__ float smem[];
     __shared__ int s_next[2];
     if ((threadIdx.x >> 6) == (blockDim.x >> 6) - 1)
         cccs_body<EXACT_G, true>(beams, n, cones, n_fwd, terma, dens_bev, partial, stride, smem, s_next);
     else
         cccs_body<EXACT_G, false>(beams, n, cones, n_fwd, terma, dens_bev, partial, stride, smem, s_next);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Workgroup = n_phi compute wavefronts + one march wavefront.
+//
+// The march of an azimuth (next lattice point, slice-end snap, midpoint voxel, edge crossings
+// and the segment-length terms shared by its 256 rays) is the same for every lane of the
+// wavefront that carries the azimuth, so computing it there costs a full wavefront instruction
+// per operation.  Here one extra wavefront marches all azimuths at once, one per lane, one chunk
+// ahead of the compute wavefronts, and hands each step over as a 32-byte record in LDS.  A
+// compute wavefront then reads one record per step (two broadcast LDS reads) and spends its
+// vector instructions on the per-ray work only.
+//
+// A chunk holds up to kRecCap steps of every azimuth and never crosses a slice end, so the one
+// barrier per chunk is the existing one per slice unless an azimuth needs more than kRecCap
+// steps in a slice.
+
+struct StepRec {
+    float gx, gy;      // lateral length per unit lattice offset: bs dz ls / sad
+    float axc, ayc;    // lateral length of the ray at offset 0
+    float dz2;         // squared longitudinal length
+    uint32_t w;        // decisions, bit fields below
+    int32_t gxw, gyw;  // general edge test: (fb & 15) | ((fn - fb) << 4) per axis
+};
+static_assert(sizeof(StepRec) == 32, "StepRec is two 16-byte LDS accesses");
+// w: [0,5) shift of the midpoint voxel along x (two's complement), [5,10) along y, [10,14) mx & 15,
+//    [14,18) my & 15, bit 18 + [19,23) voxel column whose rays crossed the x edge, bit 23 + [24,28)
+//    voxel row whose rays crossed the y edge, bit 28: a step of more than one cell (test each ray).
+constexpr int kRecCap = 32;       // steps per azimuth per chunk
+constexpr int kHdr = kMaxPhi + 2; // per chunk: step count of each azimuth, slice done, next slice
+constexpr uint32_t kColReset = 1u << 18, kRowReset = 1u << 23, kGeneralReset = 1u << 28;
+
+struct CcLds {
+    StepRec* rec;  // [2][n_phi][kRecCap]
+    float4* pos;   // [n_phi][kRecCap + 1] lattice points of the chunk being marched
+    int2* carry;   // [n_phi] (padded to 2 n_phi) midpoint voxel of the last step marched
+    float* sTR;    // [2][T | rho][256]
+    float* sC;     // [2][256]
+    float* sD;     // [2][n_phi][256]
+    int* hdr;      // [2][kHdr]
+};
+
+__host__ __device__ inline size_t march_lds_bytes(int n_phi) {
+    return (size_t)2 * n_phi * kRecCap * sizeof(StepRec) + (size_t)n_phi * (kRecCap + 1) * sizeof(float4) +
+           (size_t)2 * n_phi * sizeof(int2) + (size_t)(6 + 2 * n_phi) * kNpix * sizeof(float) +
+           (size_t)2 * kHdr * sizeof(int);
+}
+
+__device__ __forceinline__ CcLds carve_lds(float4* base, int n_phi) {
+    CcLds S;
+    S.rec = reinterpret_cast<StepRec*>(base);
+    S.pos = reinterpret_cast<float4*>(S.rec + 2 * n_phi * kRecCap);
+    S.carry = reinterpret_cast<int2*>(S.pos + n_phi * (kRecCap + 1));
+    float* f = reinterpret_cast<float*>(S.carry + 2 * n_phi);
+    S.sTR = f;
+    S.sC = f + 4 * kNpix;
+    S.sD = f + 6 * kNpix;
+    S.hdr = reinterpret_cast<int*>(f + (6 + 2 * n_phi) * kNpix);
+    return S;
+}
+
+// What every wavefront of the workgroup shares: the beamlet's slabs and output, thread layout.
+struct CcBlock {
+    const float* Tg;
+    const float* Rg;
+    float* P;
+    int L, tid, nthreads, n_phi;
+    bool two, prefetch;
+};
+
+// (32-bit offsets: a beamlet's BEV volume holds far fewer than 2^32 values)
+__device__ __forceinline__ void slab_regs(const CcBlock& K, int z, float& pf0, float& pf1) {
+    const int i0 = K.tid, i1 = K.tid + K.nthreads;
+    const uint32_t zo = (uint32_t)z * kNpix;
+    if (i0 < 2 * kNpix) pf0 = i0 < kNpix ? K.Tg[zo + i0] : K.Rg[zo + i0 - kNpix];
+    if (K.two && i1 < 2 * kNpix) pf1 = i1 < kNpix ? K.Tg[zo + i1] : K.Rg[zo + i1 - kNpix];
+}
+
+// End of a chunk, executed by every wavefront: store the prefetched slab, barrier, and after a
+// slice end the combine of the slice into P.  Returns false once the beamlet is finished.
+__device__ __forceinline__ bool chunk_end(const CcBlock& K, const CcLds& S, int done, int sn, float pf0,
+                                          float pf1, int& s, int& sbuf, int& written) {
+    if (done && sn < K.L && K.prefetch) {
+        float* nslab = S.sTR + (sbuf ^ 1) * 2 * kNpix;
+        const int i0 = K.tid, i1 = K.tid + K.nthreads;
+        if (i0 < 2 * kNpix) nslab[i0] = pf0;
+        if (K.two && i1 < 2 * kNpix) nslab[i1] = pf1;
+    }
+    __syncthreads();
+    if (!done) return true;
+    // slice s: sum over azimuths / (cumulative p of the last azimuth + 1e-4)
+    for (int i = K.tid; i < kNpix; i += K.nthreads) {
+        float sum = 0.0f;
+        for (int f = 0; f < K.n_phi; f++) sum += S.sD[(sbuf * K.n_phi + f) * kNpix + i];
+        K.P[(uint32_t)s * kNpix + i] = sum / (S.sC[sbuf * kNpix + i] + kEps);
+    }
+    for (int z = s + 1; z < sn && z < K.L; z++)  // slices the phi-0 march stepped over
+        for (int i = K.tid; i < kNpix; i += K.nthreads) K.P[(uint32_t)z * kNpix + i] = 0.0f;
+    written = sn < K.L ? sn : K.L;
+    if (sn >= K.L) return false;
+    if (!K.prefetch) {  // small workgroups load the next slab here
+        __syncthreads();
+        float* nslab = S.sTR + (sbuf ^ 1) * 2 * kNpix;
+        for (int i = K.tid; i < 2 * kNpix; i += K.nthreads)
+            nslab[i] = i < kNpix ? K.Tg[(uint32_t)sn * kNpix + i] : K.Rg[(uint32_t)sn * kNpix + i - kNpix];
+        __syncthreads();
+    }
+    s = sn;
+    sbuf ^= 1;
+    return true;
+}
+
+__device__ __forceinline__ void zero_tail(const CcBlock& K, int written) {
+    for (int z = written; z < K.L; z++)
+        for (int i = K.tid; i < kNpix; i += K.nthreads) K.P[(uint32_t)z * kNpix + i] = 0.0f;
+}
+
+// March wavefront, in two phases per chunk.  Phase 1: lane f < n_phi marches azimuth f with the
+// FP32 operation order of the CPU oracle (d_nextPoint, DoseComp.cu:76-111) and only records the
+// lattice points it reaches.  Phase 2: every lane takes one (azimuth, step) pair and derives the
+// step's record from its two end points (and the one before, for the previous midpoint voxel), so
+// the decision and segment arithmetic runs 64 steps per instruction instead of n_phi.
+struct MarchLane {
+    float dx, dy, dz, ix_, iy_, iz_, sgx, sgy, sgz;
+    float bxp, byp, bzp;
+    int nsteps;
+    bool act;
+};
+
+struct MarchConst {
+    float bsx, bsy, ls, lim_min, inv_sad;
+    int lane, kper, lf, lk;  // phase 2: lane -> azimuth lf, step lk of each group of kper steps
+};
+
+__device__ __forceinline__ void march_chunk(MarchLane& M, const MarchConst& C, const CcBlock& K, const CcLds& S,
+                                            StepRec* rec, int* hdr, int s_cur, bool new_slice) {
+    const int lane = C.lane;
+    if (new_slice) {
+        M.act = lane < K.n_phi;
+        M.nsteps = 0;
+    }
+    // phase 1: lattice points; pos[f][0] is where the chunk starts
+    float4* mypos = S.pos + (lane < K.n_phi ? lane : 0) * (kRecCap + 1);
+    if (lane < K.n_phi) mypos[0] = make_float4(M.bxp, M.byp, M.bzp, 0.0f);
+    int cnt = 0, k = 0;
+    for (; k < kRecCap; k++) {
+        if (__ballot(M.act) == 0) break;
+        if (M.act) {
+            const float sx = axis_step(M.bxp, M.sgx, M.ix_);
+            const float sy = axis_step(M.byp, M.sgy, M.iy_);
+            const float sz = axis_step(M.bzp, M.sgz, M.iz_);
+            const float st = fminf(fminf(sx, sy), sz);
+            float nx = fmaf(st, M.dx, M.bxp), ny = fmaf(st, M.dy, M.byp), nz = fmaf(st, M.dz, M.bzp);
+            // rint == round here: a tie (.5) is never within 1e-4 of an integer
+            const bool flag = fabsf(nz - rintf(nz)) < kEps;
+            if (flag) {
+                nx = fmaf(sz, M.dx, M.bxp);
+                ny = fmaf(sz, M.dy, M.byp);
+                nz = rintf(fmaf(sz, M.dz, M.bzp));
+            }
+            mypos[k + 1] = make_float4(nx, ny, nz, 0.0f);
+            M.bxp = nx;
+            M.byp = ny;
+            M.bzp = nz;
+            cnt = k + 1;
+            M.nsteps++;
+            if (flag || M.nsteps >= kMaxStepsPerSlice) M.act = false;
+        }
+    }
+    const bool done = __ballot(M.act) == 0;
+    if (lane < K.n_phi) hdr[lane] = cnt;
+    if (lane == 0) {
+        hdr[kMaxPhi] = done ? 1 : 0;
+        hdr[kMaxPhi + 1] = max((int)rintf(M.bzp), s_cur + 1);  // the phi-0 march decides the next slice
+    }
+    // phase 2: records; k is the largest step count of the chunk
+    const bool mine = C.lf < K.n_phi;
+    const int f = mine ? C.lf : 0;
+    const int c = hdr[f];
+    const float4* P = S.pos + f * (kRecCap + 1);
+    float4* out = reinterpret_cast<float4*>(rec + f * kRecCap);
+    for (int k0 = 0; k0 < k; k0 += C.kper) {
+        const int j = k0 + C.lk;
+        if (!mine || j >= c) continue;
+        const float4 p0 = P[j], p1 = P[j + 1];
+        const int mx = (int)floorf((p0.x + p1.x) * 0.5f);
+        const int my = (int)floorf((p0.y + p1.y) * 0.5f);
+        int mxp, myp;  // midpoint voxel of the previous step
+        if (j > 0) {
+            const float4 pm = P[j - 1];
+            mxp = (int)floorf((pm.x + p0.x) * 0.5f);
+            myp = (int)floorf((pm.y + p0.y) * 0.5f);
+        } else {
+            const int2 cr = S.carry[f];
+            mxp = cr.x;
+            myp = cr.y;
+        }
+        const int ddx = ((mx - mxp + 8) & 15) - 8;
+        const int ddy = ((my - myp + 8) & 15) - 8;
+        // segment length terms shared by all rays: physical lateral offset of a ray at lattice
+        // coordinate c is (c - 8) bs f(z), f(z) = (z ls + lim_min) / sad (unwrapped)
+        const float dzs = (p1.z - p0.z) * C.ls;
+        const float fe = fmaf(p1.z, C.ls, C.lim_min) * C.inv_sad;
+        const float df = dzs * C.inv_sad;
+        const float gx = C.bsx * df, gy = C.bsy * df;
+        const float axc = fmaf(gx, p0.x - 8.0f, C.bsx * (p1.x - p0.x) * fe);
+        const float ayc = fmaf(gy, p0.y - 8.0f, C.bsy * (p1.y - p0.y) * fe);
+        const int fbx = (int)floorf(p0.x), fby = (int)floorf(p0.y);
+        const int fnx = (int)floorf(p1.x), fny = (int)floorf(p1.y);
+        uint32_t w = (uint32_t)(ddx & 31) | ((uint32_t)(ddy & 31) << 5) | ((uint32_t)(mx & 15) << 10) |
+                     ((uint32_t)(my & 15) << 14);
+        // rays that cross the lateral BEV edge restart with X = 0 (DoseComp.cu:372-377); a step
+        // of at most one cell per axis crosses with one ray column and one ray row at most
+        if (abs(fnx - fbx) <= 1 && abs(fny - fby) <= 1) {
+            const int cxo = crossing_offset(fbx, fnx), cyo = crossing_offset(fby, fny);
+            if (cxo >= 0) w |= kColReset | ((uint32_t)((mx + cxo) & 15) << 19);
+            if (cyo >= 0) w |= kRowReset | ((uint32_t)((my + cyo) & 15) << 24);
+        } else {
+            w |= kGeneralReset;
+        }
+        const int gxw = (fbx & 15) | ((fnx - fbx) << 4);
+        const int gyw = (fby & 15) | ((fny - fby) << 4);
+        out[2 * j] = make_float4(gx, gy, axc, ayc);
+        out[2 * j + 1] = make_float4(dzs * dzs, __uint_as_float(w), __int_as_float(gxw), __int_as_float(gyw));
+        if (j == c - 1) S.carry[f] = make_int2(mx, my);  // after every read of this chunk's carry
+    }
+}
+
+__device__ __forceinline__ void cccs_march(const BeamletDev& B, const ConeTable* __restrict__ cones, int t,
+                                           const CcBlock& K, const CcLds& S) {
+    const int lane = threadIdx.x & 63;
+    const int f = lane < K.n_phi ? lane : 0;
+    MarchLane M;
+    {
+        // grid direction of (theta, phi_f): unit vector scaled to voxel units, renormalised
+        const float ux = cones->ux[t][f], uy = cones->uy[t][f], uz = cones->uz[t][f];
+        const float vxd = ux / B.bs[0], vyd = uy / B.bs[1], vzd = uz / B.ls;
+        const float rn = 1.0f / sqrtf(vxd * vxd + vyd * vyd + vzd * vzd);
+        M.dx = vxd * rn;
+        M.dy = vyd * rn;
+        M.dz = vzd * rn;
+        M.ix_ = 1.0f / M.dx;
+        M.iy_ = 1.0f / M.dy;
+        M.iz_ = 1.0f / M.dz;
+        M.sgx = M.dx > 0.0f ? 1.0f : -1.0f;
+        M.sgy = M.dy > 0.0f ? 1.0f : -1.0f;
+        M.sgz = M.dz > 0.0f ? 1.0f : -1.0f;
+    }
+    M.bxp = 0.5f;
+    M.byp = 0.5f;
+    M.bzp = 0.0f;
+    M.nsteps = 0;
+    M.act = false;
+    MarchConst C;
+    C.inv_sad = uniform(1.0f / B.sad);
+    C.bsx = uniform(B.bs[0]);
+    C.bsy = uniform(B.bs[1]);
+    C.ls = uniform(B.ls);
+    C.lim_min = uniform(B.lim_min);
+    C.lane = lane;
+    C.kper = 64 / K.n_phi;
+    C.lk = lane / K.n_phi;
+    C.lf = C.lk < C.kper ? lane - C.lk * K.n_phi : K.n_phi;  // n_phi: idle in phase 2
+    if (lane < K.n_phi) S.carry[lane] = make_int2(0, 0);  // the midpoint voxel starts at (0, 0)
+    const int n_phi = K.n_phi;
+
+    march_chunk(M, C, K, S, S.rec, S.hdr, 0, true);
+    __syncthreads();
+    int s = 0, buf = 0, sbuf = 0, written = 0;
+    for (;;) {
+        const int done = S.hdr[buf * kHdr + kMaxPhi], sn = S.hdr[buf * kHdr + kMaxPhi + 1];
+        float pf0 = 0.0f, pf1 = 0.0f;
+        if (done && sn < K.L && K.prefetch) slab_regs(K, sn, pf0, pf1);
+        if (!(done && sn >= K.L))
+            march_chunk(M, C, K, S, S.rec + (buf ^ 1) * n_phi * kRecCap, S.hdr + (buf ^ 1) * kHdr,
+                        done ? sn : s, done != 0);
+        if (!chunk_end(K, S, done, sn, pf0, pf1, s, sbuf, written)) break;
+        buf ^= 1;
+    }
+    zero_tail(K, written);
+}
+
+// Compute wavefront of azimuth `wave`: lane l owns voxels (x = l & 15, y = 4 (l >> 4) + r).  The
+// X states are kept pre-scaled by A/a and B/b, and the dose and cumulative length are summed in
+// units of the geometric length s (p = s rho), multiplied by rho at the end of the slice.
+template <bool EXACT_G, bool CUMU>
+__device__ __forceinline__ void cccs_compute(const ConeTable* __restrict__ cones, int t, const CcBlock& K,
+                                          const CcLds& S, int wave) {
+    const int lane = threadIdx.x & 63;
+    const float A_a = uniform(cones->A_a[t]), B_b = uniform(cones->B_b[t]);
+    const float na = uniform(-cones->a[t] * kLog2e), nb = uniform(-cones->b[t] * kLog2e);
+    const float ca = uniform(cones->a[t]), cb = uniform(cones->b[t]);
+    const int n_phi = K.n_phi;
+
+    const int vx = lane & 15;
+    const int g = lane >> 4;
+    const int vbase = vx + 64 * g;
+    const int up_lane = (lane + 48) & 63;    // lane (x, g - 1)
+    const int down_lane = (lane + 16) & 63;  // lane (x, g + 1)
+
+    float XA[kSlots], XB[kSlots];
+#pragma unroll
+    for (int r = 0; r < kSlots; r++) {
+        XA[r] = 0.0f;
+        XB[r] = 0.0f;
+    }
+    // ray offsets of my voxels relative to the midpoint voxel (mx, my), starting at (0, 0):
+    // fix = (x - mx) & 15; iy of slot r = (4 g + r - my) & 15, packed one per byte
+    float fix = (float)vx;
+    uint32_t py = (uint32_t)(4 * g) * 0x01010101u + 0x03020100u;
+    float fiy[kSlots];
+#pragma unroll
+    for (int r = 0; r < kSlots; r++) fiy[r] = (float)(4 * g + r);
+
+    __syncthreads();  // chunk 0 and slab 0 are in LDS
+    int s = 0, buf = 0, sbuf = 0, written = 0;
+    bool fresh = true;
+    float TA[kSlots], TB[kSlots], nRa[kSlots], nRb[kSlots], D[kSlots], Cs[kSlots];
+    float Rr[EXACT_G ? kSlots : 1];
+    for (;;) {
+        const int done = S.hdr[buf * kHdr + kMaxPhi], sn = S.hdr[buf * kHdr + kMaxPhi + 1];
+        float pf0 = 0.0f, pf1 = 0.0f;
+        if (done && sn < K.L && K.prefetch) slab_regs(K, sn, pf0, pf1);
+        if (fresh) {
+            const float* slab = S.sTR + sbuf * 2 * kNpix;
+#pragma unroll
+            for (int r = 0; r < kSlots; r++) {
+                const float T = slab[vbase + 16 * r], R = slab[kNpix + vbase + 16 * r];
+                TA[r] = A_a * T;
+                TB[r] = B_b * T;
+                nRa[r] = na * R;
+                nRb[r] = nb * R;
+                if (EXACT_G) Rr[r] = R;
+                D[r] = 0.0f;
+                Cs[r] = 0.0f;
+            }
+        }
+        const int cnt = __builtin_amdgcn_readfirstlane(S.hdr[buf * kHdr + wave]);
+        const float4* rp = reinterpret_cast<const float4*>(S.rec + (buf * n_phi + wave) * kRecCap);
+        // one step from its record (ra, rb)
+        auto step = [&](const float4& ra, const float4& rb) {
+            const uint32_t w = __builtin_amdgcn_readfirstlane(__float_as_uint(rb.y));
+            const int ddx = ((int)(w << 27)) >> 27;
+            const int ddy = ((int)(w << 22)) >> 27;
+            // rotate the X state (and my rays' x offsets) to follow the midpoint voxel
+            if (ddx == 1) {
+#pragma unroll
+                for (int r = 0; r < kSlots; r++) {
+                    XA[r] = row_from_left(XA[r]);
+                    XB[r] = row_from_left(XB[r]);
+                }
+                fix = row_from_left(fix);
+            } else if (ddx == -1) {
+#pragma unroll
+                for (int r = 0; r < kSlots; r++) {
+                    XA[r] = row_from_right(XA[r]);
+                    XB[r] = row_from_right(XB[r]);
+                }
+                fix = row_from_right(fix);
+            } else if (ddx != 0) {
+                shift_x(XA, XB, ddx);
+                fix = (float)((vx - (int)((w >> 10) & 15)) & 15);
+            }
+            if (ddy != 0) {
+                shift_y(XA, XB, ddy, up_lane, down_lane);
+                // (iy - ddy) & 15 per byte: iy + 16 - ddy stays within [9, 39], no carry between bytes
+                py = (py + (uint32_t)(16 - ddy) * 0x01010101u) & 0x0F0F0F0Fu;
+#pragma unroll
+                for (int r = 0; r < kSlots; r++) fiy[r] = (float)((py >> (8 * r)) & 255u);
+            }
+            const float ddxl = fmaf(ra.x, fix, ra.z);
+            const float dxz2 = fmaf(ddxl, ddxl, rb.x);
+#pragma unroll
+            for (int r = 0; r < kSlots; r++) {
+                const float ddyl = fmaf(ra.y, fiy[r], ra.w);
+                const float sl = __builtin_amdgcn_sqrtf(fmaf(ddyl, ddyl, dxz2));  // geometric length
+                // e^{-a p}, e^{-b p}, p = rho s
+                const float ea = __builtin_amdgcn_exp2f(nRa[r] * sl);
+                const float eb = __builtin_amdgcn_exp2f(nRb[r] * sl);
+                if (EXACT_G) {
+                    const float p = sl * Rr[r];
+                    const float ap = ca * p, bp = cb * p;
+                    const float ga = ap > 0.0f ? (1.0f - ea) * __builtin_amdgcn_rcpf(ap) : 1.0f;
+                    const float gb = bp > 0.0f ? (1.0f - eb) * __builtin_amdgcn_rcpf(bp) : 1.0f;
+                    D[r] = fmaf(fmaf(ga, XA[r] - TA[r], TA[r]) + fmaf(gb, XB[r] - TB[r], TB[r]), sl, D[r]);
+                } else {
+                    D[r] = fmaf(XA[r] + XB[r], sl, D[r]);
+                }
+                if (CUMU) Cs[r] += sl;
+                // X <- T (1 - e) + e X = T - e (T - X)
+                XA[r] = fmaf(-ea, TA[r] - XA[r], TA[r]);
+                XB[r] = fmaf(-eb, TB[r] - XB[r], TB[r]);
+            }
+            // rays that crossed the lateral BEV edge restart with X = 0 (DoseComp.cu:372-377)
+            if (w & kColReset) {
+                const bool m = vx == (int)((w >> 19) & 15);
+#pragma unroll
+                for (int r = 0; r < kSlots; r++) {
+                    XA[r] = m ? 0.0f : XA[r];
+                    XB[r] = m ? 0.0f : XB[r];
+                }
+            }
+            if (w & kRowReset) {
+                const int yrow = (int)((w >> 24) & 15);
+                const bool m = g == (yrow >> 2);
+                const int rr = yrow & 3;
+#pragma unroll
+                for (int r = 0; r < kSlots; r++)
+                    if (r == rr) {
+                        XA[r] = m ? 0.0f : XA[r];
+                        XB[r] = m ? 0.0f : XB[r];
+                    }
+            }
+            if (w & kGeneralReset) {
+                const int mx = (int)((w >> 10) & 15), my = (int)((w >> 14) & 15);
+                const int gxw = __builtin_amdgcn_readfirstlane(__float_as_int(rb.z));
+                const int gyw = __builtin_amdgcn_readfirstlane(__float_as_int(rb.w));
+                const int fbx = gxw & 15, fnx = fbx + (gxw >> 4);
+                const int fby = gyw & 15, fny = fby + (gyw >> 4);
+                const int ix = (vx - mx) & 15;
+                const bool cx = ((fbx + ix) >> 4) != ((fnx + ix) >> 4);
+#pragma unroll
+                for (int r = 0; r < kSlots; r++) {
+                    const int iy = (4 * g + r - my) & 15;
+                    const bool c = cx || (((fby + iy) >> 4) != ((fny + iy) >> 4));
+                    XA[r] = c ? 0.0f : XA[r];
+                    XB[r] = c ? 0.0f : XB[r];
+                }
+            }
+        };
+        // two record register sets in turn, the next record read while a step computes
+        float4 ra = rp[0], rb = rp[1], qa, qb;
+        for (int k = 0; k < cnt; k += 2) {
+            const int k1 = k + 1 < kRecCap ? k + 1 : k;
+            qa = rp[2 * k1];
+            qb = rp[2 * k1 + 1];
+            step(ra, rb);
+            if (k + 1 >= cnt) break;
+            const int k2 = k + 2 < kRecCap ? k + 2 : k;
+            ra = rp[2 * k2];
+            rb = rp[2 * k2 + 1];
+            step(qa, qb);
+        }
+        if (done) {  // publish this azimuth's slice dose (and the last azimuth's cumulative p)
+            const float* slab = S.sTR + sbuf * 2 * kNpix;
+            float* myD = S.sD + (sbuf * n_phi + wave) * kNpix;
+#pragma unroll
+            for (int r = 0; r < kSlots; r++) {
+                const float R = slab[kNpix + vbase + 16 * r];
+                myD[vbase + 16 * r] = D[r] * R;
+                if (CUMU) S.sC[sbuf * kNpix + vbase + 16 * r] = Cs[r] * R;
+            }
+        }
+        if (!chunk_end(K, S, done, sn, pf0, pf1, s, sbuf, written)) break;
+        fresh = done != 0;
+        buf ^= 1;
+    }
+    zero_tail(K, written);
+}
+
+template <bool EXACT_G>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8))) void cccs_kernel(
+    const BeamletDev* __restrict__ beams, int n, const ConeTable* __restrict__ cones, int n_fwd,
+    const float* __restrict__ terma, const float* __restrict__ dens_bev, float* __restrict__ partial,
+    uint64_t stride) {
+    extern __shared__ float4 lds4[];
+    const int t = blockIdx.x / n;  // forward-cone slot, widest cone first
+    const int b = blockIdx.x - t * n;
+    const BeamletDev& B = beams[b];
+    CcBlock K;
+    K.L = B.L;
+    if (K.L <= 0) return;
+    K.tid = threadIdx.x;
+    K.nthreads = blockDim.x;
+    K.n_phi = (blockDim.x >> 6) - 1;
+    K.Tg = terma + B.bev_off;
+    K.Rg = dens_bev + B.bev_off;
+    K.P = partial + ((uint64_t)b * n_fwd + t) * stride;
+    K.two = K.nthreads < 2 * kNpix;  // each thread prefetches two slab values
+    K.prefetch = K.nthreads >= kNpix;  // smaller workgroups reload each slab
+    const CcLds S = carve_lds(lds4, K.n_phi);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+
+    // slab of slice 0 into buffer 0 (T at [0, 256), rho at [256, 512))
+    for (int i = K.tid; i < 2 * kNpix; i += K.nthreads) S.sTR[i] = i < kNpix ? K.Tg[i] : K.Rg[i - kNpix];
+    if (wave == K.n_phi)
+        cccs_march(B, cones, t, K, S);
+    else if (wave == K.n_phi - 1)
+        cccs_compute<EXACT_G, true>(cones, t, K, S, wave);
+    else
+        cccs_compute<EXACT_G, false>(cones, t, K, S, wave);
 }
 
 // dose[b][s][p] = sum over forward cones of partial[b][t][s][p], cone slots in order.
@@ -405,14 +896,27 @@ hipError_t launch_cccs(const BeamletDev* beams, int n, int max_L, const ConeTabl
     if (n <= 0) return hipSuccess;
     if (ev) (void)hipEventRecord(ev[0], stream);
     if (n_fwd > 0) {
-        dim3 grid((unsigned)(n * n_fwd)), block(n_phi * kWave);
-        const size_t shm = (size_t)(6 + 2 * n_phi) * kNpix * sizeof(float);
-        if (exact_g)
-            hipLaunchKernelGGL(cccs_kernel<true>, grid, block, shm, stream, beams, n, cones_dev, n_fwd, terma,
-                               dens_bev, partial, stride);
-        else
-            hipLaunchKernelGGL(cccs_kernel<false>, grid, block, shm, stream, beams, n, cones_dev, n_fwd, terma,
-                               dens_bev, partial, stride);
+        dim3 grid((unsigned)(n * n_fwd));
+        // compute wavefronts + the march wavefront, within 1024 threads and 64 KiB of LDS
+        if ((n_phi + 1) * kWave <= 1024 && march_lds_bytes(n_phi) <= 65536) {
+            const dim3 block((n_phi + 1) * kWave);
+            const size_t shm = march_lds_bytes(n_phi);
+            if (exact_g)
+                hipLaunchKernelGGL(cccs_kernel<true>, grid, block, shm, stream, beams, n, cones_dev, n_fwd, terma,
+                                   dens_bev, partial, stride);
+            else
+                hipLaunchKernelGGL(cccs_kernel<false>, grid, block, shm, stream, beams, n, cones_dev, n_fwd, terma,
+                                   dens_bev, partial, stride);
+        } else {
+            const dim3 block(n_phi * kWave);
+            const size_t shm = (size_t)(6 + 2 * n_phi) * kNpix * sizeof(float);
+            if (exact_g)
+                hipLaunchKernelGGL(cccs_inline_kernel<true>, grid, block, shm, stream, beams, n, cones_dev, n_fwd,
+                                   terma, dens_bev, partial, stride);
+            else
+                hipLaunchKernelGGL(cccs_inline_kernel<false>, grid, block, shm, stream, beams, n, cones_dev, n_fwd,
+                                   terma, dens_bev, partial, stride);
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

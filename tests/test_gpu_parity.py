@@ -89,6 +89,19 @@ def test_small_exact_g(torch_cuda):
     check_against_oracle(cfg, beams, flu, g, range(len(beams)), flags=orc.EXACT_G)
 
 
+@pytest.mark.parametrize("n_phi", [2, 5, 16])
+def test_azimuth_counts(torch_cuda, n_phi):
+    """Workgroup shapes other than 8 azimuths: 2 (no slab prefetch), 5 (two slab values per
+    thread) and 16 (the kernel whose wavefronts march themselves)."""
+    cfg = synth.small_config(seed=11, dim=(22, 24, 22), fluence_dim=2, angles=(40.0, -30.0, 15.0))
+    cfg.kernel = synth.synthetic_kernel(1011, n_phi=n_phi)
+    beams, flu = beamlets_of(cfg)
+    log(f"azimuths {n_phi} ctx")
+    with fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel) as ctx:
+        g = gpu_stages(ctx, beams, flu, cfg.extent)
+    check_against_oracle(cfg, beams, flu, g, range(len(beams)))
+
+
 def test_config1_all_beamlets(torch_cuda):
     cfg = synth.config1()
     beams, flu = beamlets_of(cfg)
