@@ -72,6 +72,8 @@ ConeTable build_cones(const fd_kernel& k, int* n_fwd) {
         c.B_b[t] = k.B[i] / k.b[i];
         c.a[t] = k.a[i];
         c.b[t] = k.b[i];
+        c.na[t] = -k.a[i] * 1.4426950408889634f;
+        c.nb[t] = -k.b[i] * 1.4426950408889634f;
         const double st = std::sin((double)k.theta_mid[i]), ct = std::cos((double)k.theta_mid[i]);
         for (int f = 0; f < k.n_phi; f++) {
             const double sp = std::sin((double)k.phi[f]), cp = std::cos((double)k.phi[f]);

@@ -85,6 +85,8 @@ struct ConeTable {
     float B_b[FD_MAX_TABLE];          // B/b
     float a[FD_MAX_TABLE];
     float b[FD_MAX_TABLE];
+    float na[FD_MAX_TABLE];           // -a log2(e), -b log2(e): e^{-a p} = 2^{na p}
+    float nb[FD_MAX_TABLE];
     float ux[FD_MAX_TABLE][kMaxPhi];  // unit direction of (theta, phi), FP64-rounded
     float uy[FD_MAX_TABLE][kMaxPhi];
     float uz[FD_MAX_TABLE][kMaxPhi];

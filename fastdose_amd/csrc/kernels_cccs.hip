@@ -29,13 +29,13 @@
 // cones in a fixed order so results are bitwise reproducible.
 #include <hip/hip_runtime.h>
 
+
 #include "kernels.h"
 
 namespace fdi {
 
 namespace {
 
-constexpr float kLog2e = 1.4426950408889634f;
 constexpr int kSlots = kRaysPerLane;  // voxels (rows) per lane
 
 // Distance to the next voxel boundary along one axis, in units of the direction's parameter
@@ -138,7 +138,7 @@ __device__ __forceinline__ void cccs_body(const BeamletDev* __restrict__ beams, 
 
     // cone constants
     const float A_a = uniform(cones->A_a[t]), B_b = uniform(cones->B_b[t]);
-    const float na = uniform(-cones->a[t] * kLog2e), nb = uniform(-cones->b[t] * kLog2e);
+    const float na = cones->na[t], nb = cones->nb[t];  // scalar loads
     const float ca = uniform(cones->a[t]), cb = uniform(cones->b[t]);
 
     // grid direction of (theta, phi_wave): unit vector scaled to voxel units, renormalised
@@ -411,33 +411,43 @@ static_assert(sizeof(StepRec) == 32, "StepRec is two 16-byte LDS accesses");
 constexpr int kRecCap = 32;       // steps per azimuth per chunk
 constexpr int kHdr = kMaxPhi + 2; // per chunk: step count of each azimuth, slice done, next slice
 constexpr uint32_t kColReset = 1u << 18, kRowReset = 1u << 23, kGeneralReset = 1u << 28;
+constexpr int kMaxMarchPhi = 11;  // azimuths of the march kernel (64 KiB of LDS); more: inline kernel
+// A slab in LDS: A/a T, B/b T, -a log2(e) rho, -b log2(e) rho and rho of the slice, 256 each.
+constexpr int kSlab = 5 * kNpix;
+constexpr float kLn2 = 0.6931471805599453f;
 
 struct CcLds {
-    StepRec* rec;  // [2][n_phi][kRecCap]
+    float4* rec;   // [2][n_phi][kRecStride]: StepRec k of an azimuth at float4 2k, 2k + 1
     float4* pos;   // [n_phi][kRecCap + 1] lattice points of the chunk being marched
     int2* carry;   // [n_phi] (padded to 2 n_phi) midpoint voxel of the last step marched
-    float* sTR;    // [2][T | rho][256]
+    float* slab;   // [2][kSlab]
     float* sC;     // [2][256]
     float* sD;     // [2][n_phi][256]
     int* hdr;      // [2][kHdr]
 };
 
-__host__ __device__ inline size_t march_lds_bytes(int n_phi) {
-    return (size_t)2 * n_phi * kRecCap * sizeof(StepRec) + (size_t)n_phi * (kRecCap + 1) * sizeof(float4) +
-           (size_t)2 * n_phi * sizeof(int2) + (size_t)(6 + 2 * n_phi) * kNpix * sizeof(float) +
+// Records of one azimuth are kRecStride float4 apart (one float4 of padding spreads the azimuths
+// over the LDS banks when the march wavefront writes them).
+constexpr int kRecStride = 2 * kRecCap + 1;
+
+__host__ __device__ constexpr size_t march_lds_bytes(int n_phi) {
+    return (size_t)2 * n_phi * kRecStride * sizeof(float4) + (size_t)n_phi * (kRecCap + 1) * sizeof(float4) +
+           (size_t)2 * n_phi * sizeof(int2) + (size_t)(2 * kSlab + (2 + 2 * n_phi) * kNpix) * sizeof(float) +
            (size_t)2 * kHdr * sizeof(int);
 }
 
+static_assert(march_lds_bytes(kMaxMarchPhi) <= 65536, "march kernel LDS beyond 64 KiB");
+
 __device__ __forceinline__ CcLds carve_lds(float4* base, int n_phi) {
     CcLds S;
-    S.rec = reinterpret_cast<StepRec*>(base);
-    S.pos = reinterpret_cast<float4*>(S.rec + 2 * n_phi * kRecCap);
+    S.rec = base;
+    S.pos = base + 2 * n_phi * kRecStride;
     S.carry = reinterpret_cast<int2*>(S.pos + n_phi * (kRecCap + 1));
     float* f = reinterpret_cast<float*>(S.carry + 2 * n_phi);
-    S.sTR = f;
-    S.sC = f + 4 * kNpix;
-    S.sD = f + 6 * kNpix;
-    S.hdr = reinterpret_cast<int*>(f + (6 + 2 * n_phi) * kNpix);
+    S.slab = f;
+    S.sC = f + 2 * kSlab;
+    S.sD = S.sC + 2 * kNpix;
+    S.hdr = reinterpret_cast<int*>(S.sD + 2 * n_phi * kNpix);
     return S;
 }
 
@@ -448,33 +458,56 @@ struct CcBlock {
     float* P;
     int L, tid, nthreads, n_phi;
     bool two, prefetch;
+    float A_a, B_b, na, nb;  // cone constants the slab is scaled with
 };
 
-// (32-bit offsets: a beamlet's BEV volume holds far fewer than 2^32 values)
-__device__ __forceinline__ void slab_regs(const CcBlock& K, int z, float& pf0, float& pf1) {
-    const int i0 = K.tid, i1 = K.tid + K.nthreads;
-    const uint32_t zo = (uint32_t)z * kNpix;
-    if (i0 < 2 * kNpix) pf0 = i0 < kNpix ? K.Tg[zo + i0] : K.Rg[zo + i0 - kNpix];
-    if (K.two && i1 < 2 * kNpix) pf1 = i1 < kNpix ? K.Tg[zo + i1] : K.Rg[zo + i1 - kNpix];
+// Slab element e = 64 q + lane (q wave-uniform) of slice z: Terma for q < 4, density above, so
+// the source pointer is a scalar and only the lane offset is a vector register.
+__device__ __forceinline__ float slab_value(const CcBlock& K, int z, int q) {
+    const float* src = (q < 4 ? K.Tg : K.Rg) + (uint32_t)z * kNpix + (q & 3) * kWave;
+    return src[threadIdx.x & 63];
 }
-
+// Slab element e (e < 256: Terma, else density; wave-uniform) into the scaled LDS slab.
+__device__ __forceinline__ void store_slab(const CcBlock& K, float* slab, int e, float v) {
+    if (e < kNpix) {
+        slab[e] = K.A_a * v;
+        slab[kNpix + e] = K.B_b * v;
+    } else {
+        slab[kNpix + e] = K.na * v;  // 2 kNpix + (e - kNpix)
+        slab[2 * kNpix + e] = K.nb * v;
+        slab[3 * kNpix + e] = v;
+    }
+}
+__device__ __forceinline__ void slab_regs(const CcBlock& K, int z, float& pf0, float& pf1) {
+    const int q0 = K.tid >> 6, q1 = (K.tid + K.nthreads) >> 6;  // wave-uniform
+    if (q0 < 8) pf0 = slab_value(K, z, __builtin_amdgcn_readfirstlane(q0));
+    if (K.two && q1 < 8) pf1 = slab_value(K, z, __builtin_amdgcn_readfirstlane(q1));
+}
 // End of a chunk, executed by every wavefront: store the prefetched slab, barrier, and after a
 // slice end the combine of the slice into P.  Returns false once the beamlet is finished.
 __device__ __forceinline__ bool chunk_end(const CcBlock& K, const CcLds& S, int done, int sn, float pf0,
                                           float pf1, int& s, int& sbuf, int& written) {
     if (done && sn < K.L && K.prefetch) {
-        float* nslab = S.sTR + (sbuf ^ 1) * 2 * kNpix;
+        float* nslab = S.slab + (sbuf ^ 1) * kSlab;
         const int i0 = K.tid, i1 = K.tid + K.nthreads;
-        if (i0 < 2 * kNpix) nslab[i0] = pf0;
-        if (K.two && i1 < 2 * kNpix) nslab[i1] = pf1;
+        if (i0 < 2 * kNpix) store_slab(K, nslab, i0, pf0);
+        if (K.two && i1 < 2 * kNpix) store_slab(K, nslab, i1, pf1);
     }
     __syncthreads();
     if (!done) return true;
-    // slice s: sum over azimuths / (cumulative p of the last azimuth + 1e-4)
+    // slice s: rho sum over azimuths (in azimuth order) of the dose in units of geometric length /
+    // (rho cumulative length of the last azimuth + 1e-4); the quotient from the hardware reciprocal
     for (int i = K.tid; i < kNpix; i += K.nthreads) {
+        const float* d = S.sD + sbuf * K.n_phi * kNpix + i;
         float sum = 0.0f;
-        for (int f = 0; f < K.n_phi; f++) sum += S.sD[(sbuf * K.n_phi + f) * kNpix + i];
-        K.P[(uint32_t)s * kNpix + i] = sum / (S.sC[sbuf * kNpix + i] + kEps);
+        int f = 0;
+        for (; f + 4 <= K.n_phi; f += 4) {  // four reads in flight, summed in azimuth order
+            const float v0 = d[f * kNpix], v1 = d[(f + 1) * kNpix], v2 = d[(f + 2) * kNpix], v3 = d[(f + 3) * kNpix];
+            sum = (((sum + v0) + v1) + v2) + v3;
+        }
+        for (; f < K.n_phi; f++) sum += d[f * kNpix];
+        const float rho = S.slab[sbuf * kSlab + 4 * kNpix + i];
+        K.P[(uint32_t)s * kNpix + i] = sum * rho * __builtin_amdgcn_rcpf(fmaf(S.sC[sbuf * kNpix + i], rho, kEps));
     }
     for (int z = s + 1; z < sn && z < K.L; z++)  // slices the phi-0 march stepped over
         for (int i = K.tid; i < kNpix; i += K.nthreads) K.P[(uint32_t)z * kNpix + i] = 0.0f;
@@ -482,9 +515,9 @@ __device__ __forceinline__ bool chunk_end(const CcBlock& K, const CcLds& S, int 
     if (sn >= K.L) return false;
     if (!K.prefetch) {  // small workgroups load the next slab here
         __syncthreads();
-        float* nslab = S.sTR + (sbuf ^ 1) * 2 * kNpix;
+        float* nslab = S.slab + (sbuf ^ 1) * kSlab;
         for (int i = K.tid; i < 2 * kNpix; i += K.nthreads)
-            nslab[i] = i < kNpix ? K.Tg[(uint32_t)sn * kNpix + i] : K.Rg[(uint32_t)sn * kNpix + i - kNpix];
+            store_slab(K, nslab, i, i < kNpix ? K.Tg[(uint32_t)sn * kNpix + i] : K.Rg[(uint32_t)sn * kNpix + i - kNpix]);
         __syncthreads();
     }
     s = sn;
@@ -515,7 +548,7 @@ struct MarchConst {
 };
 
 __device__ __forceinline__ void march_chunk(MarchLane& M, const MarchConst& C, const CcBlock& K, const CcLds& S,
-                                            StepRec* rec, int* hdr, int s_cur, bool new_slice) {
+                                            float4* rec, int* hdr, int s_cur, bool new_slice) {
     const int lane = C.lane;
     if (new_slice) {
         M.act = lane < K.n_phi;
@@ -560,7 +593,7 @@ __device__ __forceinline__ void march_chunk(MarchLane& M, const MarchConst& C, c
     const int f = mine ? C.lf : 0;
     const int c = hdr[f];
     const float4* P = S.pos + f * (kRecCap + 1);
-    float4* out = reinterpret_cast<float4*>(rec + f * kRecCap);
+    float4* out = rec + f * kRecStride;
     for (int k0 = 0; k0 < k; k0 += C.kper) {
         const int j = k0 + C.lk;
         if (!mine || j >= c) continue;
@@ -654,7 +687,7 @@ __device__ __forceinline__ void cccs_march(const BeamletDev& B, const ConeTable*
         float pf0 = 0.0f, pf1 = 0.0f;
         if (done && sn < K.L && K.prefetch) slab_regs(K, sn, pf0, pf1);
         if (!(done && sn >= K.L))
-            march_chunk(M, C, K, S, S.rec + (buf ^ 1) * n_phi * kRecCap, S.hdr + (buf ^ 1) * kHdr,
+            march_chunk(M, C, K, S, S.rec + (buf ^ 1) * n_phi * kRecStride, S.hdr + (buf ^ 1) * kHdr,
                         done ? sn : s, done != 0);
         if (!chunk_end(K, S, done, sn, pf0, pf1, s, sbuf, written)) break;
         buf ^= 1;
@@ -664,14 +697,10 @@ __device__ __forceinline__ void cccs_march(const BeamletDev& B, const ConeTable*
 
 // Compute wavefront of azimuth `wave`: lane l owns voxels (x = l & 15, y = 4 (l >> 4) + r).  The
 // X states are kept pre-scaled by A/a and B/b, and the dose and cumulative length are summed in
-// units of the geometric length s (p = s rho), multiplied by rho at the end of the slice.
+// units of the geometric length s (p = s rho), multiplied by rho in the combine.
 template <bool EXACT_G, bool CUMU>
-__device__ __forceinline__ void cccs_compute(const ConeTable* __restrict__ cones, int t, const CcBlock& K,
-                                          const CcLds& S, int wave) {
+__device__ __forceinline__ void cccs_compute(const CcBlock& K, const CcLds& S, int wave) {
     const int lane = threadIdx.x & 63;
-    const float A_a = uniform(cones->A_a[t]), B_b = uniform(cones->B_b[t]);
-    const float na = uniform(-cones->a[t] * kLog2e), nb = uniform(-cones->b[t] * kLog2e);
-    const float ca = uniform(cones->a[t]), cb = uniform(cones->b[t]);
     const int n_phi = K.n_phi;
 
     const int vx = lane & 15;
@@ -698,29 +727,27 @@ __device__ __forceinline__ void cccs_compute(const ConeTable* __restrict__ cones
     int s = 0, buf = 0, sbuf = 0, written = 0;
     bool fresh = true;
     float TA[kSlots], TB[kSlots], nRa[kSlots], nRb[kSlots], D[kSlots], Cs[kSlots];
-    float Rr[EXACT_G ? kSlots : 1];
     for (;;) {
         const int done = S.hdr[buf * kHdr + kMaxPhi], sn = S.hdr[buf * kHdr + kMaxPhi + 1];
         float pf0 = 0.0f, pf1 = 0.0f;
         if (done && sn < K.L && K.prefetch) slab_regs(K, sn, pf0, pf1);
         if (fresh) {
-            const float* slab = S.sTR + sbuf * 2 * kNpix;
+            const float* slab = S.slab + sbuf * kSlab + vbase;
 #pragma unroll
             for (int r = 0; r < kSlots; r++) {
-                const float T = slab[vbase + 16 * r], R = slab[kNpix + vbase + 16 * r];
-                TA[r] = A_a * T;
-                TB[r] = B_b * T;
-                nRa[r] = na * R;
-                nRb[r] = nb * R;
-                if (EXACT_G) Rr[r] = R;
+                TA[r] = slab[16 * r];
+                TB[r] = slab[kNpix + 16 * r];
+                nRa[r] = slab[2 * kNpix + 16 * r];
+                nRb[r] = slab[3 * kNpix + 16 * r];
                 D[r] = 0.0f;
                 Cs[r] = 0.0f;
             }
         }
         const int cnt = __builtin_amdgcn_readfirstlane(S.hdr[buf * kHdr + wave]);
-        const float4* rp = reinterpret_cast<const float4*>(S.rec + (buf * n_phi + wave) * kRecCap);
+        const float4* rp = S.rec + (buf * n_phi + wave) * kRecStride;
+        const float2* rp2 = reinterpret_cast<const float2*>(rp);  // (dz2, w) of record k at 4 k + 2
         // one step from its record (ra, rb)
-        auto step = [&](const float4& ra, const float4& rb) {
+        auto step = [&](const float4& ra, const float2& rb, int k) {
             const uint32_t w = __builtin_amdgcn_readfirstlane(__float_as_uint(rb.y));
             const int ddx = ((int)(w << 27)) >> 27;
             const int ddy = ((int)(w << 22)) >> 27;
@@ -760,8 +787,8 @@ __device__ __forceinline__ void cccs_compute(const ConeTable* __restrict__ cones
                 const float ea = __builtin_amdgcn_exp2f(nRa[r] * sl);
                 const float eb = __builtin_amdgcn_exp2f(nRb[r] * sl);
                 if (EXACT_G) {
-                    const float p = sl * Rr[r];
-                    const float ap = ca * p, bp = cb * p;
+                    // a p = a rho s = -ln(2) (-a log2(e) rho s)
+                    const float ap = -kLn2 * (nRa[r] * sl), bp = -kLn2 * (nRb[r] * sl);
                     const float ga = ap > 0.0f ? (1.0f - ea) * __builtin_amdgcn_rcpf(ap) : 1.0f;
                     const float gb = bp > 0.0f ? (1.0f - eb) * __builtin_amdgcn_rcpf(bp) : 1.0f;
                     D[r] = fmaf(fmaf(ga, XA[r] - TA[r], TA[r]) + fmaf(gb, XB[r] - TB[r], TB[r]), sl, D[r]);
@@ -795,8 +822,9 @@ __device__ __forceinline__ void cccs_compute(const ConeTable* __restrict__ cones
             }
             if (w & kGeneralReset) {
                 const int mx = (int)((w >> 10) & 15), my = (int)((w >> 14) & 15);
-                const int gxw = __builtin_amdgcn_readfirstlane(__float_as_int(rb.z));
-                const int gyw = __builtin_amdgcn_readfirstlane(__float_as_int(rb.w));
+                const float2 gw = rp2[4 * k + 3];
+                const int gxw = __builtin_amdgcn_readfirstlane(__float_as_int(gw.x));
+                const int gyw = __builtin_amdgcn_readfirstlane(__float_as_int(gw.y));
                 const int fbx = gxw & 15, fnx = fbx + (gxw >> 4);
                 const int fby = gyw & 15, fny = fby + (gyw >> 4);
                 const int ix = (vx - mx) & 15;
@@ -811,26 +839,25 @@ __device__ __forceinline__ void cccs_compute(const ConeTable* __restrict__ cones
             }
         };
         // two record register sets in turn, the next record read while a step computes
-        float4 ra = rp[0], rb = rp[1], qa, qb;
+        float4 ra = rp[0], qa;
+        float2 rb = rp2[2], qb;
         for (int k = 0; k < cnt; k += 2) {
             const int k1 = k + 1 < kRecCap ? k + 1 : k;
             qa = rp[2 * k1];
-            qb = rp[2 * k1 + 1];
-            step(ra, rb);
+            qb = rp2[4 * k1 + 2];
+            step(ra, rb, k);
             if (k + 1 >= cnt) break;
             const int k2 = k + 2 < kRecCap ? k + 2 : k;
             ra = rp[2 * k2];
-            rb = rp[2 * k2 + 1];
-            step(qa, qb);
+            rb = rp2[4 * k2 + 2];
+            step(qa, qb, k + 1);
         }
-        if (done) {  // publish this azimuth's slice dose (and the last azimuth's cumulative p)
-            const float* slab = S.sTR + sbuf * 2 * kNpix;
+        if (done) {  // publish this azimuth's slice dose (and the last azimuth's cumulative length)
             float* myD = S.sD + (sbuf * n_phi + wave) * kNpix;
 #pragma unroll
             for (int r = 0; r < kSlots; r++) {
-                const float R = slab[kNpix + vbase + 16 * r];
-                myD[vbase + 16 * r] = D[r] * R;
-                if (CUMU) S.sC[sbuf * kNpix + vbase + 16 * r] = Cs[r] * R;
+                myD[vbase + 16 * r] = D[r];
+                if (CUMU) S.sC[sbuf * kNpix + vbase + 16 * r] = Cs[r];
             }
         }
         if (!chunk_end(K, S, done, sn, pf0, pf1, s, sbuf, written)) break;
@@ -840,6 +867,8 @@ __device__ __forceinline__ void cccs_compute(const ConeTable* __restrict__ cones
     zero_tail(K, written);
 }
 
+// 80 VGPRs (6 wavefronts per SIMD) keep two workgroups of 9 wavefronts on a CU; with 96 only one
+// fits.
 template <bool EXACT_G>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8))) void cccs_kernel(
     const BeamletDev* __restrict__ beams, int n, const ConeTable* __restrict__ cones, int n_fwd,
@@ -863,14 +892,18 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8))) vo
     const CcLds S = carve_lds(lds4, K.n_phi);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
-    // slab of slice 0 into buffer 0 (T at [0, 256), rho at [256, 512))
-    for (int i = K.tid; i < 2 * kNpix; i += K.nthreads) S.sTR[i] = i < kNpix ? K.Tg[i] : K.Rg[i - kNpix];
+    K.A_a = cones->A_a[t];
+    K.B_b = cones->B_b[t];
+    K.na = cones->na[t];
+    K.nb = cones->nb[t];
+    // slab of slice 0 into buffer 0
+    for (int i = K.tid; i < 2 * kNpix; i += K.nthreads) store_slab(K, S.slab, i, i < kNpix ? K.Tg[i] : K.Rg[i - kNpix]);
     if (wave == K.n_phi)
         cccs_march(B, cones, t, K, S);
     else if (wave == K.n_phi - 1)
-        cccs_compute<EXACT_G, true>(cones, t, K, S, wave);
+        cccs_compute<EXACT_G, true>(K, S, wave);
     else
-        cccs_compute<EXACT_G, false>(cones, t, K, S, wave);
+        cccs_compute<EXACT_G, false>(K, S, wave);
 }
 
 // dose[b][s][p] = sum over forward cones of partial[b][t][s][p], cone slots in order.
@@ -898,7 +931,7 @@ hipError_t launch_cccs(const BeamletDev* beams, int n, int max_L, const ConeTabl
     if (n_fwd > 0) {
         dim3 grid((unsigned)(n * n_fwd));
         // compute wavefronts + the march wavefront, within 1024 threads and 64 KiB of LDS
-        if ((n_phi + 1) * kWave <= 1024 && march_lds_bytes(n_phi) <= 65536) {
+        if (n_phi <= kMaxMarchPhi) {
             const dim3 block((n_phi + 1) * kWave);
             const size_t shm = march_lds_bytes(n_phi);
             if (exact_g)
