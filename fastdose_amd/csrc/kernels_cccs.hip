@@ -411,7 +411,7 @@ static_assert(sizeof(StepRec) == 32, "StepRec is two 16-byte LDS accesses");
 constexpr int kRecCap = 32;       // steps per azimuth per chunk
 constexpr int kHdr = kMaxPhi + 2; // per chunk: step count of each azimuth, slice done, next slice
 constexpr uint32_t kColReset = 1u << 18, kRowReset = 1u << 23, kGeneralReset = 1u << 28;
-constexpr int kMaxMarchPhi = 11;  // azimuths of the march kernel (64 KiB of LDS); more: inline kernel
+constexpr int kMaxMarchPhi = 10;  // azimuths of the march kernel (64 KiB of LDS); more: inline kernel
 // A slab in LDS: A/a T, B/b T, -a log2(e) rho, -b log2(e) rho and rho of the slice, 256 each.
 constexpr int kSlab = 5 * kNpix;
 constexpr float kLn2 = 0.6931471805599453f;
@@ -420,9 +420,10 @@ struct CcLds {
     float4* rec;   // [2][n_phi][kRecStride]: StepRec k of an azimuth at float4 2k, 2k + 1
     float4* pos;   // [n_phi][kRecCap + 1] lattice points of the chunk being marched
     int2* carry;   // [n_phi] (padded to 2 n_phi) midpoint voxel of the last step marched
-    float* slab;   // [2][kSlab]
-    float* sC;     // [2][256]
-    float* sD;     // [2][n_phi][256]
+    float* slab;   // [3][kSlab], slice i in buffer i % 3 (the combine of slice i - 1 still reads its rho
+                   // while slice i + 1 is being prefetched)
+    float* sC;     // [2][256], slice i in buffer i & 1
+    float* sD;     // [2][n_phi][256], slice i in buffer i & 1
     int* hdr;      // [2][kHdr]
 };
 
@@ -432,7 +433,7 @@ constexpr int kRecStride = 2 * kRecCap + 1;
 
 __host__ __device__ constexpr size_t march_lds_bytes(int n_phi) {
     return (size_t)2 * n_phi * kRecStride * sizeof(float4) + (size_t)n_phi * (kRecCap + 1) * sizeof(float4) +
-           (size_t)2 * n_phi * sizeof(int2) + (size_t)(2 * kSlab + (2 + 2 * n_phi) * kNpix) * sizeof(float) +
+           (size_t)2 * n_phi * sizeof(int2) + (size_t)(3 * kSlab + (2 + 2 * n_phi) * kNpix) * sizeof(float) +
            (size_t)2 * kHdr * sizeof(int);
 }
 
@@ -445,7 +446,7 @@ __device__ __forceinline__ CcLds carve_lds(float4* base, int n_phi) {
     S.carry = reinterpret_cast<int2*>(S.pos + n_phi * (kRecCap + 1));
     float* f = reinterpret_cast<float*>(S.carry + 2 * n_phi);
     S.slab = f;
-    S.sC = f + 2 * kSlab;
+    S.sC = f + 3 * kSlab;
     S.sD = S.sC + 2 * kNpix;
     S.hdr = reinterpret_cast<int*>(S.sD + 2 * n_phi * kNpix);
     return S;
@@ -486,9 +487,9 @@ __device__ __forceinline__ void slab_regs(const CcBlock& K, int z, float& pf0, f
 // End of a chunk, executed by every wavefront: store the prefetched slab, barrier, and after a
 // slice end the combine of the slice into P.  Returns false once the beamlet is finished.
 __device__ __forceinline__ bool chunk_end(const CcBlock& K, const CcLds& S, int done, int sn, float pf0,
-                                          float pf1, int& s, int& sbuf, int& written) {
+                                          float pf1, int& s, int& si, int& written) {
     if (done && sn < K.L && K.prefetch) {
-        float* nslab = S.slab + (sbuf ^ 1) * kSlab;
+        float* nslab = S.slab + ((si + 1) % 3) * kSlab;
         const int i0 = K.tid, i1 = K.tid + K.nthreads;
         if (i0 < 2 * kNpix) store_slab(K, nslab, i0, pf0);
         if (K.two && i1 < 2 * kNpix) store_slab(K, nslab, i1, pf1);
@@ -498,7 +499,7 @@ __device__ __forceinline__ bool chunk_end(const CcBlock& K, const CcLds& S, int 
     // slice s: rho sum over azimuths (in azimuth order) of the dose in units of geometric length /
     // (rho cumulative length of the last azimuth + 1e-4); the quotient from the hardware reciprocal
     for (int i = K.tid; i < kNpix; i += K.nthreads) {
-        const float* d = S.sD + sbuf * K.n_phi * kNpix + i;
+        const float* d = S.sD + (si & 1) * K.n_phi * kNpix + i;
         float sum = 0.0f;
         int f = 0;
         for (; f + 4 <= K.n_phi; f += 4) {  // four reads in flight, summed in azimuth order
@@ -506,8 +507,8 @@ __device__ __forceinline__ bool chunk_end(const CcBlock& K, const CcLds& S, int 
             sum = (((sum + v0) + v1) + v2) + v3;
         }
         for (; f < K.n_phi; f++) sum += d[f * kNpix];
-        const float rho = S.slab[sbuf * kSlab + 4 * kNpix + i];
-        K.P[(uint32_t)s * kNpix + i] = sum * rho * __builtin_amdgcn_rcpf(fmaf(S.sC[sbuf * kNpix + i], rho, kEps));
+        const float rho = S.slab[(si % 3) * kSlab + 4 * kNpix + i];
+        K.P[(uint32_t)s * kNpix + i] = sum * rho * __builtin_amdgcn_rcpf(fmaf(S.sC[(si & 1) * kNpix + i], rho, kEps));
     }
     for (int z = s + 1; z < sn && z < K.L; z++)  // slices the phi-0 march stepped over
         for (int i = K.tid; i < kNpix; i += K.nthreads) K.P[(uint32_t)z * kNpix + i] = 0.0f;
@@ -515,13 +516,13 @@ __device__ __forceinline__ bool chunk_end(const CcBlock& K, const CcLds& S, int 
     if (sn >= K.L) return false;
     if (!K.prefetch) {  // small workgroups load the next slab here
         __syncthreads();
-        float* nslab = S.slab + (sbuf ^ 1) * kSlab;
+        float* nslab = S.slab + ((si + 1) % 3) * kSlab;
         for (int i = K.tid; i < 2 * kNpix; i += K.nthreads)
             store_slab(K, nslab, i, i < kNpix ? K.Tg[(uint32_t)sn * kNpix + i] : K.Rg[(uint32_t)sn * kNpix + i - kNpix]);
         __syncthreads();
     }
     s = sn;
-    sbuf ^= 1;
+    si++;
     return true;
 }
 
@@ -681,7 +682,7 @@ __device__ __forceinline__ void cccs_march(const BeamletDev& B, const ConeTable*
 
     march_chunk(M, C, K, S, S.rec, S.hdr, 0, true);
     __syncthreads();
-    int s = 0, buf = 0, sbuf = 0, written = 0;
+    int s = 0, buf = 0, si = 0, written = 0;  // si: ordinal of the slice being processed
     for (;;) {
         const int done = S.hdr[buf * kHdr + kMaxPhi], sn = S.hdr[buf * kHdr + kMaxPhi + 1];
         float pf0 = 0.0f, pf1 = 0.0f;
@@ -689,7 +690,7 @@ __device__ __forceinline__ void cccs_march(const BeamletDev& B, const ConeTable*
         if (!(done && sn >= K.L))
             march_chunk(M, C, K, S, S.rec + (buf ^ 1) * n_phi * kRecStride, S.hdr + (buf ^ 1) * kHdr,
                         done ? sn : s, done != 0);
-        if (!chunk_end(K, S, done, sn, pf0, pf1, s, sbuf, written)) break;
+        if (!chunk_end(K, S, done, sn, pf0, pf1, s, si, written)) break;
         buf ^= 1;
     }
     zero_tail(K, written);
@@ -724,7 +725,7 @@ __device__ __forceinline__ void cccs_compute(const CcBlock& K, const CcLds& S, i
     for (int r = 0; r < kSlots; r++) fiy[r] = (float)(4 * g + r);
 
     __syncthreads();  // chunk 0 and slab 0 are in LDS
-    int s = 0, buf = 0, sbuf = 0, written = 0;
+    int s = 0, buf = 0, si = 0, written = 0;  // si: ordinal of the slice being processed
     bool fresh = true;
     float TA[kSlots], TB[kSlots], nRa[kSlots], nRb[kSlots], D[kSlots], Cs[kSlots];
     for (;;) {
@@ -732,7 +733,7 @@ __device__ __forceinline__ void cccs_compute(const CcBlock& K, const CcLds& S, i
         float pf0 = 0.0f, pf1 = 0.0f;
         if (done && sn < K.L && K.prefetch) slab_regs(K, sn, pf0, pf1);
         if (fresh) {
-            const float* slab = S.slab + sbuf * kSlab + vbase;
+            const float* slab = S.slab + (si % 3) * kSlab + vbase;
 #pragma unroll
             for (int r = 0; r < kSlots; r++) {
                 TA[r] = slab[16 * r];
@@ -853,14 +854,14 @@ __device__ __forceinline__ void cccs_compute(const CcBlock& K, const CcLds& S, i
             step(qa, qb, k + 1);
         }
         if (done) {  // publish this azimuth's slice dose (and the last azimuth's cumulative length)
-            float* myD = S.sD + (sbuf * n_phi + wave) * kNpix;
+            float* myD = S.sD + ((si & 1) * n_phi + wave) * kNpix;
 #pragma unroll
             for (int r = 0; r < kSlots; r++) {
                 myD[vbase + 16 * r] = D[r];
-                if (CUMU) S.sC[sbuf * kNpix + vbase + 16 * r] = Cs[r];
+                if (CUMU) S.sC[(si & 1) * kNpix + vbase + 16 * r] = Cs[r];
             }
         }
-        if (!chunk_end(K, S, done, sn, pf0, pf1, s, sbuf, written)) break;
+        if (!chunk_end(K, S, done, sn, pf0, pf1, s, si, written)) break;
         fresh = done != 0;
         buf ^= 1;
     }

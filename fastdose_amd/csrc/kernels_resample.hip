@@ -177,16 +177,23 @@ __device__ float voxel_value(const BeamletDev& B, const float* __restrict__ dose
     // IEEE division.  The rotation's partial products are hoisted out of the sample loops (same
     // expression, same rounding).
     constexpr float kNearInt = 4e-5f;
+    // Range tests as floor-to-integer plus one unsigned compare each: with d clamped to [-1, L] and
+    // u, w to [-1, 16] (which keeps every decision), d in [0, L) <=> floor(d) in [0, L) and u in
+    // [0, 16) <=> floor(u) in [0, 16), and floor equals the truncation the index needs there.  A
+    // sample that fails reads element 0 and adds 0, so the five samples of a column issue their
+    // loads without branches.
+    const uint32_t Lu = (uint32_t)B.L;
     float acc = 0.0f;
     for (int ii = 0; ii < kResampleSS; ii++) {
         const float px = M[0] * qx[ii], py = M[3] * qx[ii], pz = M[6] * qx[ii];
         for (int jj = 0; jj < kResampleSS; jj++) {
             const float ax = fmaf(M[1], qy[jj], px), ay = fmaf(M[4], qy[jj], py), az = fmaf(M[7], qy[jj], pz);
+            float val[kResampleSS];
 #pragma unroll
             for (int kk = 0; kk < kResampleSS; kk++) {
                 const float by = fmaf(M[5], qz[kk], ay);
                 const float dd = (by - B.lim_min) * B.inv_ls;
-                if (!(dd >= 0.0f && dd < Lf)) continue;
+                const int id = (int)floorf(fminf(fmaxf(dd, -1.0f), Lf));
                 const float bx = fmaf(M[2], qz[kk], ax);
                 const float bz = fmaf(M[8], qz[kk], az);
                 const float tx = bx * B.kx, tz = bz * B.kz;
@@ -199,9 +206,15 @@ __device__ float voxel_value(const BeamletDev& B, const float* __restrict__ dose
                     u = fmaf(tx, rinv, half);
                     w = fmaf(tz, rinv, half);
                 }
-                if (!(u >= 0.0f && u < (float)kFmap && w >= 0.0f && w < (float)kFmap)) continue;
-                acc += D[(size_t)(int)dd * kNpix + (int)w * kFmap + (int)u];
+                const int iu = (int)floorf(fminf(fmaxf(u, -1.0f), (float)kFmap));
+                const int iw = (int)floorf(fminf(fmaxf(w, -1.0f), (float)kFmap));
+                const bool ok = (uint32_t)id < Lu && (uint32_t)iu < (uint32_t)kFmap && (uint32_t)iw < (uint32_t)kFmap;
+                const uint32_t off = ok ? ((uint32_t)id * kNpix + (uint32_t)iw * kFmap + (uint32_t)iu) : 0u;
+                const float v = D[off];
+                val[kk] = ok ? v : 0.0f;
             }
+#pragma unroll
+            for (int kk = 0; kk < kResampleSS; kk++) acc += val[kk];
         }
     }
     return acc * (1.0f / (kResampleSS * kResampleSS * kResampleSS));
