@@ -177,12 +177,14 @@ __device__ float voxel_value(const BeamletDev& B, const float* __restrict__ dose
     // IEEE division.  The rotation's partial products are hoisted out of the sample loops (same
     // expression, same rounding).
     constexpr float kNearInt = 4e-5f;
-    // Range tests as floor-to-integer plus one unsigned compare each: with d clamped to [-1, L] and
-    // u, w to [-1, 16] (which keeps every decision), d in [0, L) <=> floor(d) in [0, L) and u in
-    // [0, 16) <=> floor(u) in [0, 16), and floor equals the truncation the index needs there.  A
-    // sample that fails reads element 0 and adds 0, so the five samples of a column issue their
-    // loads without branches.
+    // The range tests are the oracle's float compares, combined without branches (the compare
+    // masks are scalar).  The BEV dose is read through a buffer resource of the beamlet's L x 256
+    // values: a sample that fails reads beyond the range and gets 0 from the hardware, so the five
+    // samples of a column issue their loads together and add exactly what the oracle adds.
     const uint32_t Lu = (uint32_t)B.L;
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void*)D, (short)0, (int)(Lu * kNpix * sizeof(float)), 0x00020000);
+    constexpr uint32_t kOutside = 0x80000000u;  // byte offset beyond any beamlet's range
     float acc = 0.0f;
     for (int ii = 0; ii < kResampleSS; ii++) {
         const float px = M[0] * qx[ii], py = M[3] * qx[ii], pz = M[6] * qx[ii];
@@ -193,25 +195,25 @@ __device__ float voxel_value(const BeamletDev& B, const float* __restrict__ dose
             for (int kk = 0; kk < kResampleSS; kk++) {
                 const float by = fmaf(M[5], qz[kk], ay);
                 const float dd = (by - B.lim_min) * B.inv_ls;
-                const int id = (int)floorf(fminf(fmaxf(dd, -1.0f), Lf));
+                const bool okd = (dd >= 0.0f) & (dd < Lf);
                 const float bx = fmaf(M[2], qz[kk], ax);
                 const float bz = fmaf(M[8], qz[kk], az);
                 const float tx = bx * B.kx, tz = bz * B.kz;
                 const float ra = __builtin_amdgcn_rcpf(by);
                 float u = fmaf(tx, ra, half);
                 float w = fmaf(tz, ra, half);
-                const float fu = __builtin_amdgcn_fractf(u), fw = __builtin_amdgcn_fractf(w);
-                if (fu < kNearInt || fu > 1.0f - kNearInt || fw < kNearInt || fw > 1.0f - kNearInt) {
+                // near an integer (|u - n| < kNearInt, w alike): take the IEEE quotient
+                const float nu = __builtin_amdgcn_fractf(u + kNearInt), nw = __builtin_amdgcn_fractf(w + kNearInt);
+                if (fminf(nu, nw) < 2.0f * kNearInt) {
                     const float rinv = 1.0f / by;
                     u = fmaf(tx, rinv, half);
                     w = fmaf(tz, rinv, half);
                 }
-                const int iu = (int)floorf(fminf(fmaxf(u, -1.0f), (float)kFmap));
-                const int iw = (int)floorf(fminf(fmaxf(w, -1.0f), (float)kFmap));
-                const bool ok = (uint32_t)id < Lu && (uint32_t)iu < (uint32_t)kFmap && (uint32_t)iw < (uint32_t)kFmap;
-                const uint32_t off = ok ? ((uint32_t)id * kNpix + (uint32_t)iw * kFmap + (uint32_t)iu) : 0u;
-                const float v = D[off];
-                val[kk] = ok ? v : 0.0f;
+                const bool ok = okd & (u >= 0.0f) & (u < (float)kFmap) & (w >= 0.0f) & (w < (float)kFmap);
+                // truncation == floor on the accepted range; the other arm is never selected
+                const uint32_t boff =
+                    ok ? ((uint32_t)(int)dd * kNpix + (uint32_t)(int)w * kFmap + (uint32_t)(int)u) * 4u : kOutside;
+                val[kk] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)boff, 0, 0));
             }
 #pragma unroll
             for (int kk = 0; kk < kResampleSS; kk++) acc += val[kk];
