@@ -135,6 +135,7 @@ struct fd_ctx {
     int device = 0;
     fd_volume vol{};
     float* density = nullptr;
+    float* density_pad = nullptr;  // density with a one-voxel zero border in 4 x 4 x 4 bricks (Terma)
     fd_spectrum spectrum{};
     fd_kernel kernel{};
     fdi::SpectrumDev spec_dev{};
@@ -178,7 +179,7 @@ int max_long_dim(const fd_beam* beams, int n);
 // Single stages on a workspace (descriptors uploaded by upload_beams first).
 int upload_beams(Workspace& w, const fd_beam* beams, int n, uint64_t stride, hipStream_t stream);
 int run_terma(const fd_ctx* ctx, Workspace& w, int options, int n, const float* fluence, float* terma, float* dens,
-              hipStream_t stream);
+              uint64_t stride, hipStream_t stream);
 int run_cccs(const fd_ctx* ctx, Workspace& w, int options, int n, int max_L, const float* terma, const float* dens,
              float* dose, uint64_t stride, hipStream_t stream);
 // Resampler stages 1-2 with exact sizing (two host synchronisations); leaves the offsets in w.

@@ -146,7 +146,7 @@ size_t batch_footprint(const fd_ctx* ctx, const fd_beam* beams, int n) {
     double cand = 0;
     for (int i = 0; i < n; i++) cand += frustum_voxels(beams[i], ctx->vol);
     const size_t segs = (size_t)n * ctx->vol.dim[2];
-    return (size_t)n * stride * sizeof(float) * (3 + std::max(ctx->n_fwd, 1)) + (size_t)n * kNpix * sizeof(float) +
+    return (size_t)n * stride * sizeof(float) * (3 + std::max(ctx->n_fwd, 4)) + (size_t)n * kNpix * sizeof(float) +
            (size_t)(2.0 * cand) * (2 * sizeof(uint64_t) + 2 * sizeof(float)) + segs * 24 + (size_t)n * 128;
 }
 
@@ -159,16 +159,20 @@ int upload_beams(Workspace& w, const fd_beam* beams, int n, uint64_t stride, hip
     return FD_OK;
 }
 
+int reserve_cccs(const fd_ctx* ctx, Workspace& w, int n, uint64_t stride);
+
 int run_terma(const fd_ctx* ctx, Workspace& w, int options, int n, const float* fluence, float* terma, float* dens,
-              hipStream_t stream) {
+              uint64_t stride, hipStream_t stream) {
     // FP64 radiological path on rays longer than kLongRayVoxels voxels (or on all rays)
     const float vmin = std::min(ctx->vol.voxel_size[0], std::min(ctx->vol.voxel_size[1], ctx->vol.voxel_size[2]));
     const float rpl64_len = (options & FD_OPT_RPL_FP64_ALL) ? -1.0f
                             : (options & FD_OPT_RPL_FP64) ? kLongRayVoxels * vmin
                                                           : INFINITY;
+    // the per-cone partial buffer (4 floats per BEV value at least) is free until the CCCS stage
+    FD_TRY(reserve_cccs(ctx, w, n, stride));
     w.mark(FD_T_TERMA, 0, stream);
-    hipError_t e = launch_terma(w.desc.ptr, n, fluence, ctx->density, ctx->vol, ctx->spec_dev, rpl64_len, terma,
-                                dens, stream);
+    hipError_t e = launch_terma(w.desc.ptr, n, fluence, ctx->density_pad, ctx->vol, ctx->spec_dev, rpl64_len, terma,
+                                dens, w.partial.ptr, stream);
     w.mark(FD_T_TERMA, 1, stream);
     if (e != hipSuccess) {
         set_error(std::string("terma launch: ") + hipGetErrorString(e));
@@ -178,7 +182,7 @@ int run_terma(const fd_ctx* ctx, Workspace& w, int options, int n, const float* 
 }
 
 int reserve_cccs(const fd_ctx* ctx, Workspace& w, int n, uint64_t stride) {
-    FD_TRY(w.partial.reserve((size_t)n * std::max(ctx->n_fwd, 1) * stride));
+    FD_TRY(w.partial.reserve((size_t)n * std::max(ctx->n_fwd, 4) * stride));  // also the Terma scratch
     return FD_OK;
 }
 
@@ -354,7 +358,7 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
         FD_HIP(hipMemcpyAsync(w.desc.ptr, w.stage, (size_t)n * sizeof(BeamletDev), hipMemcpyHostToDevice, st));
         FD_HIP(hipMemcpyAsync(w.fluence.ptr, w.flu_stage, (size_t)n * kNpix * sizeof(float), hipMemcpyHostToDevice,
                               st));
-        FD_TRY(run_terma(ctx, w, options, n, w.fluence.ptr, w.terma.ptr, w.dens_bev.ptr, st));
+        FD_TRY(run_terma(ctx, w, options, n, w.fluence.ptr, w.terma.ptr, w.dens_bev.ptr, stride, st));
         FD_TRY(run_cccs(ctx, w, options, n, maxL, w.terma.ptr, w.dens_bev.ptr, w.dose_bev.ptr, stride, st));
         FD_TRY(launch_count(ctx, w, n, st));
         FD_TRY(launch_values(ctx, w, n, w.dose_bev.ptr, extent, thr, cap, st));
