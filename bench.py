@@ -51,7 +51,10 @@ def profiled_traffic(name: str):
     return int(float(m.group(1))) if m else None
 
 
-def cpu_baseline(cfg, beams, flu, sample: int):
+def cpu_baseline(cfg, beams, flu, sample: int, gpu_rows=None, n_check: int = 4):
+    """The FP64 oracle on the host cores: its beamlets/s on a bounded sample, and (gpu_rows given) the
+    GPU rows of n_check of those beamlets checked against it: max |GPU - FP64| / beamlet max and
+    whether the column sets agree."""
     import numpy as np
 
     import oracle as orc
@@ -65,7 +68,7 @@ def cpu_baseline(cfg, beams, flu, sample: int):
                                orc.cast(orc.Spectrum, cfg.spectrum), orc.cast(orc.Kernel, cfg.kernel), cfg.extent, 0,
                                threads)
     dt = time.perf_counter() - t0
-    return {
+    out = {
         "value": round(sample / dt, 3),
         "unit": "beamlets/s",
         "cores": threads,
@@ -73,6 +76,25 @@ def cpu_baseline(cfg, beams, flu, sample: int):
         "sample": f"{sample} of the beam's {len(beams)} beamlets (every {len(beams) // sample}th), full FP64 path "
                   f"Terma->CCCS->resample, OpenMP over beamlets, {dt:.1f} s wall",
     }
+    if gpu_rows is not None:
+        offsets, cols, vals = gpu_rows
+        worst, same_cols = 0.0, True
+        check = idx[:: max(1, len(idx) // n_check)][:n_check]
+        for i in check:
+            ref = orc.beamlet_pipeline(orc.cast(orc.Beam, beams[i]), flu[i], orc.cast(orc.Volume, cfg.volume),
+                                       cfg.density, orc.cast(orc.Spectrum, cfg.spectrum),
+                                       orc.cast(orc.Kernel, cfg.kernel), cfg.extent, 0)
+            a, b = int(offsets[i]), int(offsets[i + 1])
+            gc, gv = cols[a:b].astype(np.int64), vals[a:b].astype(np.float64)
+            u = np.union1d(gc, ref["cols"])
+            dg, dr = np.zeros(len(u)), np.zeros(len(u))
+            dg[np.searchsorted(u, gc)] = gv
+            dr[np.searchsorted(u, ref["cols"])] = ref["vals"]
+            worst = max(worst, float(np.abs(dg - dr).max() / dr.max()))
+            same_cols = same_cols and np.array_equal(gc, ref["cols"])
+        out["accuracy_vs_fp64"] = {"beamlets": [int(i) for i in check], "max_err_rel_beamlet_max": worst,
+                                   "columns_equal": bool(same_cols)}
+    return out
 
 
 def device_flags(ctx, cfg, torch):
@@ -123,7 +145,8 @@ class BeamStore:
 
 def leg_batched_4pi(fd, synth, torch, np):
     """Batched 4pi workload: 180 beams on a 200x200x160 body, 4 isocenters, 20x20 fluence, beamlets
-    that intersect the PTV; plain one-beam-at-a-time path vs all beams batched."""
+    that intersect the PTV; plain one-beam-at-a-time path (default launch setting) vs all beams batched
+    with the setting fd_autotune picks on half of the beams."""
     t0 = time.perf_counter()
     cfg = synth.fourpi_config("batched-4pi-200x200x160", (200, 200, 160), 1003, 180, (22.0, 15.0, 13.0),
                               with_flags=False)
@@ -152,7 +175,12 @@ def leg_batched_4pi(fd, synth, torch, np):
     plain_s = time.perf_counter() - t
     nnz_plain = dm.info()[2]
     dm.close()
-    # batched: every beam through one launch per stage (chunks of 8192 beamlets)
+    # autotune the batched launch on every other beam (setup, not timed), then batch every beam
+    t = time.perf_counter()
+    sample = list(range(0, len(cfg.bundles), 2))
+    sa, sf, sc = store.request(sample)
+    tuning = ctx.autotune(sa, sf, sc, cfg.extent)
+    log(f"batched 4pi: autotune {tuning} in {time.perf_counter() - t:.1f} s")
     arr, flu, cnt = store.request(range(len(cfg.bundles)))
     dm = fd.DoseMat(nvox)
     torch.cuda.synchronize()
@@ -169,6 +197,7 @@ def leg_batched_4pi(fd, synth, torch, np):
         "plain_beam_latency_ms": {"p50": round(percentile(lat, 50), 3), "p99": round(percentile(lat, 99), 3)},
         "batched_beamlets_per_s": round(n / batched_s, 1),
         "batched_over_plain": round(plain_s / batched_s, 3),
+        "tuning": {k: (round(v, 1) if isinstance(v, float) else v) for k, v in tuning.items()},
         "nnz": int(nnz_b),
     })
     log(f"batched 4pi: plain {n / plain_s:.0f} beamlets/s, batched {n / batched_s:.0f} beamlets/s")
@@ -256,12 +285,21 @@ def leg_spmm(fd, synth, torch, np, cfg, k=256, reps=10):
     """Precision leg: thresholded (1e-3) dose-loading matrix of the slab beam times 256 fluence vectors,
     FP32 / BF16 / FP16 values; HBM-bound (writing Y dominates)."""
     ctx = fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel)
+    # reference: the unthresholded FP32 matrix times the same vectors
+    full = fd.dose_mat_construction(ctx, cfg.bundles, cfg.flags, cfg.extent)
     ctx.set_threshold(1e-3)
     dm = fd.dose_mat_construction(ctx, cfg.bundles, cfg.flags, cfg.extent)
     _, rows, nnz = dm.info()
     ncols = dm.n_cols
     X = torch.rand((rows, k), dtype=torch.float32, device="cuda", generator=torch.Generator("cuda").manual_seed(4))
-    out = {"rows": rows, "cols": ncols, "nnz": nnz, "k": k, "types": {}}
+    pf = fd.Spmm(full, fd.FD_VAL_F32)
+    Yref = pf.run(X)
+    torch.cuda.synchronize()
+    ymax = float(Yref.abs().max())
+    pf.close()
+    full.close()
+    out = {"rows": rows, "cols": ncols, "nnz": nnz, "k": k, "types": {},
+           "error_reference": "unthresholded FP32 matrix, same vectors; error = max |Y - Y_ref| / max |Y_ref|"}
     for name, vt, vb in (("f32", fd.FD_VAL_F32, 4), ("bf16", fd.FD_VAL_BF16, 2), ("fp16", fd.FD_VAL_FP16, 2)):
         p = fd.Spmm(dm, vt)
         Y = p.run(X)
@@ -274,7 +312,8 @@ def leg_spmm(fd, synth, torch, np, cfg, k=256, reps=10):
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / reps
         bytes_alg = ncols * k * 4 + nnz * (vb + 4) + (ncols + 1) * 8 + rows * k * 4
-        out["types"][name] = {"ms": round(ms, 4), "GB_per_s": round(bytes_alg / ms / 1e6, 1)}
+        err = float((Y - Yref).abs().max()) / ymax
+        out["types"][name] = {"ms": round(ms, 4), "GB_per_s": round(bytes_alg / ms / 1e6, 1), "max_err_rel": err}
         p.close()
     dm.close()
     ctx.close()
@@ -412,6 +451,7 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
+    gpu_rows = (offsets.cpu().numpy(), cols[0][:nnz].cpu().numpy(), vals[0][:nnz].cpu().numpy())
     ctx.close()
     if not args.no_extras:
         out["batched_4pi"], cap = leg_batched_4pi(fd, synth, torch, np)
@@ -419,7 +459,7 @@ def main() -> int:
         out["spmm_precision"] = leg_spmm(fd, synth, torch, np, cfg)
     if not args.no_cpu_baseline:
         log(f"cpu baseline: oracle on {args.cpu_sample} beamlets")
-        out["cpu_baseline"] = cpu_baseline(cfg, beams, flu, args.cpu_sample)
+        out["cpu_baseline"] = cpu_baseline(cfg, beams, flu, args.cpu_sample, gpu_rows)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -90,6 +90,13 @@ class ServeStats(C.Structure):
                 ("peak_inflight_bytes", C.c_uint64), ("peak_inflight_batches", C.c_int)]
 
 
+class Tuning(C.Structure):
+    """fd_tuning: launch parameters of the batched paths (fd_autotune)."""
+
+    _fields_ = [("batch_beamlets", C.c_int), ("graph", C.c_int), ("terma_segments", C.c_int),
+                ("beamlets_per_s", C.c_float)]
+
+
 P = C.POINTER
 vp = C.c_void_p
 u8p = P(C.c_uint8)
@@ -135,6 +142,9 @@ SIGNATURES = {
     "fd_spmm_info": (C.c_int, [vp, P(C.c_uint64), P(C.c_uint64), P(C.c_uint64)]),
     "fd_spmm_run": (C.c_int, [vp, vp, C.c_int, vp, vp]),
     "fd_spmm_destroy": (C.c_int, [vp]),
+    "fd_ctx_get_tuning": (C.c_int, [vp, P(Tuning)]),
+    "fd_ctx_set_tuning": (C.c_int, [vp, P(Tuning)]),
+    "fd_autotune": (C.c_int, [vp, P(Beam), fp, P(C.c_int), C.c_int, C.c_float, P(Tuning), vp]),
     "fd_serve_start": (C.c_int, [vp, P(ServeOpts), P(vp)]),
     "fd_serve_submit": (C.c_int, [vp, P(Beam), fp, P(C.c_int), C.c_int, P(C.c_uint64)]),
     "fd_serve_poll": (C.c_int, [vp, C.c_uint64, P(C.c_int)]),

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -722,7 +723,8 @@ int fd_dosemat_add_beams(fd_ctx* ctx, fd_dosemat* m, const fd_beam* beamlets, co
         set_error("dose matrix column count differs from the context's voxel count");
         return FD_ERR_ARG;
     }
-    if (max_chunk <= 0) max_chunk = 8192;
+    if (max_chunk <= 0) max_chunk = ctx->tune.batch_beamlets;
+    const int options = ctx->options | (ctx->tune.graph ? FD_OPT_GRAPH : 0);
     hipStream_t s = (hipStream_t)stream;
     std::vector<uint64_t> host_off;
     BatchResult res;
@@ -732,8 +734,8 @@ int fd_dosemat_add_beams(fd_ctx* ctx, fd_dosemat* m, const fd_beam* beamlets, co
         int k1 = k0;
         uint64_t n = 0;
         while (k1 < n_beams && (k1 == k0 || n + (uint64_t)counts[k1] <= (uint64_t)max_chunk)) n += counts[k1++];
-        FD_TRY(run_batch(ctx, ctx->ws, ctx->options, beamlets + first[k0], (int)n, fluence + first[k0] * kNpix,
-                         extent, ctx->threshold, s, &res));
+        FD_TRY(run_batch(ctx, ctx->ws, options, beamlets + first[k0], (int)n, fluence + first[k0] * kNpix, extent,
+                         ctx->threshold, s, &res));
         // one matrix per beam: offsets re-based to each beam's first row
         host_off.clear();
         host_off.reserve(n + (k1 - k0));
@@ -774,6 +776,100 @@ int fd_dosemat_add_beam(fd_ctx* ctx, fd_dosemat* m, const fd_beam* beamlets, int
         return FD_ERR_ARG;
     }
     return fd_dosemat_add_beams(ctx, m, beamlets, fluence, &n, 1, n, extent, stream);
+}
+
+int fd_ctx_get_tuning(const fd_ctx* ctx, fd_tuning* out) {
+    if (!ctx || !out) {
+        set_error("null argument");
+        return FD_ERR_ARG;
+    }
+    *out = ctx->tune;
+    return FD_OK;
+}
+
+int fd_ctx_set_tuning(fd_ctx* ctx, const fd_tuning* t) {
+    FD_TRY(check_ctx(ctx));
+    if (!t || t->batch_beamlets < 1 || (t->graph != 0 && t->graph != 1) ||
+        (t->terma_segments != 1 && t->terma_segments != 2 && t->terma_segments != 4)) {
+        set_error("tuning: batch_beamlets >= 1, graph 0 or 1, terma_segments 1, 2 or 4");
+        return FD_ERR_ARG;
+    }
+    ctx->tune = *t;
+    return FD_OK;
+}
+
+int fd_autotune(fd_ctx* ctx, const fd_beam* beamlets, const float* fluence, const int* counts, int n_beams,
+                float extent, fd_tuning* out, void* stream) {
+    FD_TRY(check_ctx(ctx));
+    if (!out || n_beams <= 0 || !counts) {
+        set_error("autotune needs beams and an output");
+        return FD_ERR_ARG;
+    }
+    uint64_t total = 0;
+    for (int k = 0; k < n_beams; k++) total += counts[k] > 0 ? (uint64_t)counts[k] : 0;
+    if (total == 0) {
+        set_error("autotune needs at least one beamlet");
+        return FD_ERR_ARG;
+    }
+    const uint64_t nvox = (uint64_t)ctx->vol.dim[0] * ctx->vol.dim[1] * ctx->vol.dim[2];
+    const fd_tuning saved = ctx->tune;
+    // one setting, run twice on a fresh matrix each time; beamlets per second of the faster run
+    auto trial = [&](const fd_tuning& t, double& bps) -> int {
+        ctx->tune = t;
+        double best_s = 1e30;
+        for (int rep = 0; rep < 2; rep++) {
+            fd_dosemat* m = nullptr;
+            int rc = fd_dosemat_create(nvox, 0, &m);
+            if (rc != FD_OK) return rc;
+            const auto t0 = std::chrono::steady_clock::now();
+            rc = fd_dosemat_add_beams(ctx, m, beamlets, fluence, counts, n_beams, 0, extent, stream);
+            if (rc == FD_OK && hipStreamSynchronize((hipStream_t)stream) != hipSuccess) rc = FD_ERR_HIP;
+            const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            fd_dosemat_destroy(m);
+            if (rc != FD_OK) return rc;
+            best_s = std::min(best_s, sec);
+        }
+        bps = (double)total / best_s;
+        return FD_OK;
+    };
+    fd_tuning best = saved;
+    best.beamlets_per_s = 0.0f;
+    double best_bps = 0.0, bps = 0.0;
+    int rc = FD_OK;
+    auto consider = [&](fd_tuning t) {
+        if (rc != FD_OK) return;
+        rc = trial(t, bps);
+        if (rc == FD_OK && bps > best_bps) {
+            best_bps = bps;
+            best = t;
+        }
+    };
+    consider(best);  // also warms up the workspace
+    for (int b : {1024, 2048, 4096, 8192, 16384}) {
+        if (b == best.batch_beamlets || (uint64_t)b >= 2 * total + 1024) continue;
+        fd_tuning t = best;
+        t.batch_beamlets = b;
+        consider(t);
+    }
+    {
+        fd_tuning t = best;
+        t.graph = 1 - t.graph;
+        consider(t);
+    }
+    for (int g : {2, 4}) {
+        if (g == best.terma_segments) continue;
+        fd_tuning t = best;
+        t.terma_segments = g;
+        consider(t);
+    }
+    if (rc != FD_OK) {
+        ctx->tune = saved;
+        return rc;
+    }
+    best.beamlets_per_s = (float)best_bps;
+    ctx->tune = best;
+    *out = best;
+    return FD_OK;
 }
 
 int fd_dosemat_add_csr(fd_dosemat* m, const fd_csr* csr, void* stream) {

@@ -172,7 +172,7 @@ int run_terma(const fd_ctx* ctx, Workspace& w, int options, int n, const float* 
     FD_TRY(reserve_cccs(ctx, w, n, stride));
     w.mark(FD_T_TERMA, 0, stream);
     hipError_t e = launch_terma(w.desc.ptr, n, fluence, ctx->density_pad, ctx->vol, ctx->spec_dev, rpl64_len, terma,
-                                dens, w.partial.ptr, stream);
+                                dens, w.partial.ptr, ctx->tune.terma_segments, stream);
     w.mark(FD_T_TERMA, 1, stream);
     if (e != hipSuccess) {
         set_error(std::string("terma launch: ") + hipGetErrorString(e));

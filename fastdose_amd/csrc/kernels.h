@@ -16,10 +16,11 @@ hipError_t launch_beamlet_flags(const FlagBeamDev* beams, int n, int max_pixels,
 // density_pad: the density with a one-voxel zero border, in 4 x 4 x 4 bricks: padded voxel
 // (x, y, z) at ((z/4 nby + y/4) nbx + x/4) 64 + 16 (z%4) + 4 (y%4) + x%4, nb* = ceil((dim + 2) / 4).
 // scratch: 4 floats per BEV value (the layout of terma, 16-byte aligned), overwritten.
+// segments: ray segments per workgroup (1, 2 or 4; 256 threads each).
 // Beamlets with long_dim * long_spacing > rpl64_len [cm] accumulate the radiological path in FP64.
 hipError_t launch_terma(const BeamletDev* beams, int n, const float* fluence, const float* density_pad,
                         const fd_volume& vol, const SpectrumDev& spec, float rpl64_len, float* terma,
-                        float* dens_bev, float* scratch, hipStream_t stream);
+                        float* dens_bev, float* scratch, int segments, hipStream_t stream);
 
 // CCCS over the forward cones (DoseComp.cu:217-392): per-cone partial BEV doses
 // partial[(b * n_fwd + t) * stride + s * 256 + p], then their sum into dose.

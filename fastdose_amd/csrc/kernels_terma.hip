@@ -13,15 +13,14 @@
 // through a buffer resource, so that the eight corners need no bounds tests.
 //
 // The radiological path is a prefix sum along each ray, which one lane per ray would walk alone
-// (256 rays per beamlet: one wavefront per SIMD on small batches).  Each ray is cut into
-// kTermaSeg segments of slices, one wavefront group per segment: pass 1 writes the mean density
-// of its slices, sums rho |step| over its segment and (FP32 path) stores the running in-segment
-// sums of the four samples of each slice in a scratch buffer; after a barrier each segment adds
-// the sum of the segments before it (in order) and pass 2 reads the stored sums back for the
-// Terma.  (The FP64 path walks its samples again instead of storing doubles.)  Segment 0
-// reproduces the sequential accumulation exactly; later segments differ from it by the rounding
-// of one regrouped sum.  Rays with zero fluence skip pass 2's spectrum sum: their
-// Terma is exactly 0.
+// (256 rays per beamlet: one wavefront per SIMD on small batches).  Each ray is cut into 1, 2 or
+// 4 segments of slices (a launch parameter), one wavefront group per segment: pass 1 writes the
+// mean density of its slices, sums rho |step| over its segment and (FP32 path) stores the running
+// in-segment sums of the four samples of each slice in a scratch buffer; after a barrier each
+// segment adds the sum of the segments before it (in order) and pass 2 reads the stored sums back
+// for the Terma.  (The FP64 path walks its samples again instead of storing doubles.)  Segment 0
+// reproduces the sequential accumulation exactly; later segments differ from it by the rounding of
+// one regrouped sum.  Rays with zero fluence skip pass 2's spectrum sum: their Terma is exactly 0.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -30,7 +29,7 @@ namespace fdi {
 
 namespace {
 
-constexpr int kTermaSeg = 4;  // ray segments per beamlet (wavefront groups of the workgroup)
+constexpr int kTermaSeg = 4;  // most ray segments per beamlet (wavefront groups of the workgroup)
 
 // Trilinear density at voxel-unit position (cx, cy, cz) (texel centres at i + 0.5, zero outside
 // the volume) from the zero-bordered, bricked copy (nbx, nby: bricks along x and y).
@@ -72,6 +71,7 @@ __global__ __launch_bounds__(kTermaSeg * kNpix) void terma_kernel(
     const int b = blockIdx.x;
     const int p = threadIdx.x & (kNpix - 1);
     const int seg = __builtin_amdgcn_readfirstlane(threadIdx.x / kNpix);
+    const int nseg = blockDim.x / kNpix;
     const BeamletDev& B = beams[b];
     const int ix = p & (kFmap - 1), iy = p >> 4;
     const float half = 0.5f * kFmap;
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kTermaSeg * kNpix) void terma_kernel(
     float* __restrict__ R = dens_bev + B.bev_off + p;
     float4* __restrict__ Q = scratch + B.bev_off + p;  // running in-segment sums, 4 per slice
     const bool RPL64 = (float)B.L * B.ls > rpl64_len;  // uniform over the workgroup
-    const int i0 = (seg * B.L) / kTermaSeg, i1 = ((seg + 1) * B.L) / kTermaSeg;
+    const int i0 = (seg * B.L) / nseg, i1 = ((seg + 1) * B.L) / nseg;
 
     // pass 1: mean density per slice, radiological length of the segment
     float lr = 0.0f;
@@ -177,9 +177,10 @@ __global__ __launch_bounds__(kTermaSeg * kNpix) void terma_kernel(
 
 hipError_t launch_terma(const BeamletDev* beams, int n, const float* fluence, const float* density_pad,
                         const fd_volume& vol, const SpectrumDev& spec, float rpl64_len, float* terma,
-                        float* dens_bev, float* scratch, hipStream_t stream) {
+                        float* dens_bev, float* scratch, int segments, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    dim3 grid(n), block(kTermaSeg * kNpix);
+    if (segments != 1 && segments != 2 && segments != kTermaSeg) return hipErrorInvalidValue;
+    dim3 grid(n), block(segments * kNpix);
     const int X = vol.dim[0], Y = vol.dim[1], Z = vol.dim[2];
     hipLaunchKernelGGL(terma_kernel, grid, block, 0, stream, beams, fluence, density_pad, X, Y, Z, vol.voxel_size[0],
                        vol.voxel_size[1], vol.voxel_size[2], spec, rpl64_len, terma, dens_bev,

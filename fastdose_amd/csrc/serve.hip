@@ -245,7 +245,9 @@ int fd_serve_start(fd_ctx* ctx, const fd_serve_opts* opts, fd_server** out) {
         return FD_ERR_ARG;
     }
     if (o.n_slots == 0) o.n_slots = 3;
-    if (o.max_batch_beamlets == 0) o.max_batch_beamlets = 4096;
+    if (o.max_batch_beamlets == 0)  // the context's tuned batch size, if it was tuned
+        o.max_batch_beamlets = ctx->tune.beamlets_per_s > 0.0f ? ctx->tune.batch_beamlets : 4096;
+    if (ctx->tune.graph) o.options |= FD_OPT_GRAPH;
     if (o.extent == 0.0f) o.extent = 2.0f;
     FD_HIP(hipSetDevice(ctx->device));
     if (o.hbm_budget == 0) {

@@ -128,6 +128,31 @@ class Context:
         """Keep CSR entries >= rel x (the beamlet's maximum); 0 drops exact zeros only."""
         call("fd_ctx_set_threshold", self._h, rel)
 
+    @property
+    def tuning(self) -> dict:
+        """Launch parameters of the batched paths (batch_beamlets, graph, terma_segments, beamlets_per_s)."""
+        t = _lib.Tuning()
+        call("fd_ctx_get_tuning", self._h, C.byref(t))
+        return {f: getattr(t, f) for f, _ in t._fields_}
+
+    def set_tuning(self, batch_beamlets: int, graph: int, terma_segments: int) -> None:
+        t = _lib.Tuning(batch_beamlets, graph, terma_segments, 0.0)
+        call("fd_ctx_set_tuning", self._h, C.byref(t))
+
+    def autotune(self, beamlets, fluence: np.ndarray, counts, extent: float = 2.0, stream=None) -> dict:
+        """Time trial batches of these beams (fd_autotune) and keep the fastest launch setting."""
+        arr = beam_array(beamlets)
+        cnt = np.ascontiguousarray(counts, dtype=np.int32)
+        if int(cnt.sum()) != len(arr):
+            raise ValueError("counts must add up to the number of beamlets")
+        flu = np.ascontiguousarray(fluence, dtype=np.float32)
+        if flu.size != len(arr) * NPIX:
+            raise ValueError("fluence must hold 256 floats per beamlet")
+        t = _lib.Tuning()
+        call("fd_autotune", self._h, arr, _f32(flu), cnt.ctypes.data_as(C.POINTER(C.c_int)), len(cnt), extent,
+             C.byref(t), _stream_arg(stream))
+        return {f: getattr(t, f) for f, _ in t._fields_}
+
     def timings(self) -> dict:
         """Per-kernel milliseconds of the last launches (needs FD_OPT_TIMING)."""
         n = len(_lib.TIMING_SLOTS)

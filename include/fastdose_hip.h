@@ -244,6 +244,26 @@ int fd_dosemat_add_beam(fd_ctx* ctx, fd_dosemat* m, const fd_beam* beamlets, int
  * identical to what fd_dosemat_add_beam gives for it alone. */
 int fd_dosemat_add_beams(fd_ctx* ctx, fd_dosemat* m, const fd_beam* beamlets, const float* fluence,
                          const int* counts, int n_beams, int max_chunk, float extent, void* stream);
+
+/* ---- Launch tuning (no counterpart in the reference, which launches one beam at a time with
+ * fixed shapes): the parameters of the batched paths, chosen by fd_autotune. ---- */
+typedef struct fd_tuning {
+    int batch_beamlets;   /* beamlets per batch when fd_dosemat_add_beams / serving are given 0 */
+    int graph;            /* 1: replay each batch as one captured HIP graph (FD_OPT_GRAPH) */
+    int terma_segments;   /* ray segments per Terma workgroup (wavefront groups of 256 rays): 1, 2, 4 */
+    float beamlets_per_s; /* throughput of this setting in the tuning run (0: defaults, not tuned) */
+} fd_tuning;
+
+int fd_ctx_get_tuning(const fd_ctx* ctx, fd_tuning* out);
+int fd_ctx_set_tuning(fd_ctx* ctx, const fd_tuning* t);
+
+/* Times trial runs of fd_dosemat_add_beams over the given beams (whole beams in order; a few dozen
+ * representative beams suffice): batch sizes of 1024 .. 16384 beamlets, then graph replay on and
+ * off, then 2 or 4 Terma segments, each setting run twice and its faster run kept.  The fastest
+ * setting becomes the context's tuning and is returned in *out.  Synchronous; the trial matrices
+ * are discarded. */
+int fd_autotune(fd_ctx* ctx, const fd_beam* beamlets, const float* fluence, const int* counts, int n_beams,
+                float extent, fd_tuning* out, void* stream);
 /* Appends an existing device CSR (copied). */
 int fd_dosemat_add_csr(fd_dosemat* m, const fd_csr* csr, void* stream);
 int fd_dosemat_info(const fd_dosemat* m, uint64_t* n_matrices, uint64_t* total_rows, uint64_t* total_nnz);
@@ -296,7 +316,7 @@ typedef struct fd_server fd_server;
 
 typedef struct fd_serve_opts {
     int n_slots;                 /* concurrent batches (streams); 0: 3 */
-    int max_batch_beamlets;      /* beamlets per batch; 0: 4096 */
+    int max_batch_beamlets;      /* beamlets per batch; 0: the context's tuning (fd_autotune), else 4096 */
     uint64_t hbm_budget;         /* bytes of batch scratch in flight; 0: half the free HBM */
     float extent;                /* resampler extent [cm]; 0: 2.0 */
     int options;                 /* FD_OPT_EXACT_G / FD_OPT_RPL_FP64(_ALL) for the batches */

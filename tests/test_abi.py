@@ -41,6 +41,7 @@ def test_struct_layouts_match_header():
     assert C.sizeof(fd.CSR) == 48
     assert C.sizeof(fd._lib.ServeOpts) == 32
     assert C.sizeof(fd._lib.ServeStats) == 4 * 8 + 6 * 8 + 8 + 8
+    assert C.sizeof(fd._lib.Tuning) == 16
 
 
 def test_version_and_error_strings():

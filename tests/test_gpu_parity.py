@@ -235,6 +235,49 @@ def test_batched_beams_match_per_beam(torch_cuda):
         assert ref.matrix(2)[0].tolist() == [0]
 
 
+@pytest.mark.parametrize("segments", [1, 2, 4])
+def test_terma_segments(torch_cuda, segments):
+    """Terma with 1, 2 or 4 ray segments per workgroup (a tuning parameter) stays within the bar."""
+    cfg = synth.small_config(seed=21, dim=(26, 24, 30), fluence_dim=2, angles=(15.0, 50.0, -20.0))
+    beams, flu = beamlets_of(cfg)
+    with fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel) as ctx:
+        ctx.set_tuning(8192, 0, segments)
+        assert ctx.tuning["terma_segments"] == segments
+        g = gpu_stages(ctx, beams, flu, cfg.extent)
+    check_against_oracle(cfg, beams, flu, g, range(len(beams)))
+
+
+def test_autotune(torch_cuda):
+    """fd_autotune picks a valid setting, keeps it in the context, and batched construction with it
+    gives each beam's per-beam matrix (same context, same setting) bit for bit."""
+    cfg = synth.small_config(seed=22, dim=(28, 26, 24), fluence_dim=3)
+    iso = cfg.bundles[0].isocenter
+    for ang in ((200.0, 10.0, 5.0), (75.0, -40.0, 0.0), (310.0, 80.0, 30.0), (120.0, 20.0, 60.0)):
+        cfg.bundles.append(synth.make_bundle(iso, ang, fluence_dim=3))
+        cfg.flags.append(np.ones(9, dtype=np.uint8))
+    beamlets, fls, counts = [], [], []
+    for b, f in zip(cfg.bundles, cfg.flags):
+        bl, fl = fd.beamlets_init(b, f, cfg.volume)
+        beamlets.extend(bl)
+        fls.append(fl)
+        counts.append(len(bl))
+    with fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel) as ctx:
+        with pytest.raises(fd.FastDoseError):
+            ctx.set_tuning(0, 0, 4)
+        with pytest.raises(fd.FastDoseError):
+            ctx.set_tuning(1024, 0, 3)
+        t = ctx.autotune(beamlets, np.concatenate(fls, 0), counts, cfg.extent)
+        log(f"autotune {t}")
+        assert t["batch_beamlets"] in (1024, 2048, 4096, 8192, 16384) and t["graph"] in (0, 1)
+        assert t["terma_segments"] in (2, 4) and t["beamlets_per_s"] > 0
+        assert ctx.tuning == t
+        ref = fd.dose_mat_construction(ctx, cfg.bundles, cfg.flags, cfg.extent)
+        dm = fd.dose_mat_construction(ctx, cfg.bundles, cfg.flags, cfg.extent, batched=True, max_chunk=0)
+        assert dm.info() == ref.info()
+        for k in range(len(cfg.bundles)):
+            assert all(np.array_equal(x, y) for x, y in zip(ref.matrix(k), dm.matrix(k))), k
+
+
 def test_device_beamlet_flags_match_host(torch_cuda):
     """fd_beamlet_flags (device, many bundles) == fd_beamlet_flags_host, bit for bit."""
     torch = torch_cuda
