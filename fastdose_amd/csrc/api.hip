@@ -378,30 +378,13 @@ int fd_ctx_create(int device, const fd_volume* vol, const float* density_host, c
     if ((e = hipMemcpy(c->density, density_host, nvox * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "density upload");
     {
-        // zero-bordered copy for the Terma ray trace (trilinear corners need no bounds tests), in
-        // 4 x 4 x 4 bricks so that rays of any direction share cache lines
-        const size_t X = vol->dim[0], Y = vol->dim[1], Z = vol->dim[2];
-        const size_t Xp = X + 2, Yp = Y + 2, Zp = Z + 2;
-        const size_t nbx = (Xp + 3) / 4, nby = (Yp + 3) / 4, nbz = (Zp + 3) / 4;
-        const size_t nb = nbx * nby * nbz * 64;
-        if (nb * sizeof(float) > (size_t)INT32_MAX) {  // one buffer resource addresses it
-            set_error("volume too large: the bricked density must stay below 2^29 values");
-            fd_ctx_destroy(c);
-            return FD_ERR_UNSUPPORTED;
-        }
-        std::vector<float> pad(nb, 0.0f);
-        for (size_t z = 0; z < Z; z++)
-            for (size_t y = 0; y < Y; y++)
-                for (size_t x = 0; x < X; x++) {
-                    const size_t px = x + 1, py = y + 1, pz = z + 1;
-                    const size_t brick = ((pz >> 2) * nby + (py >> 2)) * nbx + (px >> 2);
-                    pad[brick * 64 + (pz & 3) * 16 + (py & 3) * 4 + (px & 3)] = density_host[(z * Y + y) * X + x];
-                }
-        if ((e = hipMalloc((void**)&c->density_pad, pad.size() * sizeof(float))) != hipSuccess)
-            return fail(e, "hipMalloc padded density");
-        if ((e = hipMemcpy(c->density_pad, pad.data(), pad.size() * sizeof(float), hipMemcpyHostToDevice)) !=
-            hipSuccess)
-            return fail(e, "padded density upload");
+        // per-voxel trilinear corner records for the Terma ray trace (built on the device)
+        const size_t nrec = density_records_floats(*vol);
+        if ((e = hipMalloc((void**)&c->density_pad, nrec * sizeof(float))) != hipSuccess)
+            return fail(e, "hipMalloc density corner records");
+        if ((e = launch_expand_density(c->density, *vol, c->density_pad, nullptr)) != hipSuccess ||
+            (e = hipDeviceSynchronize()) != hipSuccess)
+            return fail(e, "density corner records");
     }
     if ((e = hipMalloc((void**)&c->spec_tab, tab.size() * sizeof(float))) != hipSuccess)
         return fail(e, "hipMalloc spectrum table");

@@ -135,7 +135,7 @@ struct fd_ctx {
     int device = 0;
     fd_volume vol{};
     float* density = nullptr;
-    float* density_pad = nullptr;  // density with a one-voxel zero border in 4 x 4 x 4 bricks (Terma)
+    float* density_pad = nullptr;  // per-voxel trilinear corner records of the density (Terma)
     fd_spectrum spectrum{};
     fd_kernel kernel{};
     fdi::SpectrumDev spec_dev{};

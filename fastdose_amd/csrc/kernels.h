@@ -12,9 +12,13 @@ namespace fdi {
 hipError_t launch_beamlet_flags(const FlagBeamDev* beams, int n, int max_pixels, const uint8_t* mask,
                                 const fd_volume& vol, uint8_t* flags, hipStream_t stream);
 
+// Per-voxel trilinear corner records of the density (8 floats per base voxel of the zero-bordered
+// volume, 4 x 4 x 4 bricks) for the Terma kernel; density_records_floats(vol) floats.
+uint64_t density_records_floats(const fd_volume& vol);
+hipError_t launch_expand_density(const float* density, const fd_volume& vol, float* records, hipStream_t stream);
+
 // Terma + mean density along the 256 BEV rays of each beamlet (TermaComp.cu:179-253).
-// density_pad: the density with a one-voxel zero border, in 4 x 4 x 4 bricks: padded voxel
-// (x, y, z) at ((z/4 nby + y/4) nbx + x/4) 64 + 16 (z%4) + 4 (y%4) + x%4, nb* = ceil((dim + 2) / 4).
+// density_pad: the corner records of launch_expand_density.
 // scratch: 4 floats per BEV value (the layout of terma, 16-byte aligned), overwritten.
 // segments: ray segments per workgroup (1, 2 or 4; 256 threads each).
 // Beamlets with long_dim * long_spacing > rpl64_len [cm] accumulate the radiological path in FP64.

@@ -8,9 +8,10 @@
 // and the density output is the substep mean of rho.  The spectrum sum is read from a table over
 // the radiological depth built with the context (linear interpolation, error below ~5e-6).  The
 // density is sampled with a hand-written trilinear filter (texel centres at i + 0.5, zero outside
-// the volume) instead of a texture unit, reading a zero-bordered copy of the density (4 x 4 x 4
-// bricks, so that the 64 rays of a wavefront share cache lines whatever the beam direction)
-// through a buffer resource, so that the eight corners need no bounds tests.
+// the volume) instead of a texture unit.  It reads per-voxel corner records built once per context
+// (the eight corners of the voxel's trilinear cell with a zero border, in 4 x 4 x 4 bricks): one
+// 32-byte read per sample instead of eight gathers, no bounds tests, and rays of any direction
+// share cache lines.
 //
 // The radiological path is a prefix sum along each ray, which one lane per ray would walk alone
 // (256 rays per beamlet: one wavefront per SIMD on small batches).  Each ray is cut into 1, 2 or
@@ -31,30 +32,48 @@ namespace {
 
 constexpr int kTermaSeg = 4;  // most ray segments per beamlet (wavefront groups of the workgroup)
 
+// Corner records: for every base voxel (x, y, z) in [0, X] x [0, Y] x [0, Z] of the density with a
+// one-voxel zero border, its eight trilinear corners (x..x+1, y..y+1, z..z+1; x fastest) as two
+// float4, the records in 4 x 4 x 4 bricks: record ((z/4 nby + y/4) nbx + x/4) 64 + 16 (z%4) +
+// 4 (y%4) + x%4, nb* = ceil((dim + 1) / 4).  A sample reads one 32-byte record.
+__device__ __forceinline__ uint64_t corner_record(uint32_t x, uint32_t y, uint32_t z, uint32_t nbx, uint32_t nby) {
+    return ((((uint64_t)(z >> 2) * nby + (y >> 2)) * nbx + (x >> 2)) << 6) + ((z & 3) << 4) + ((y & 3) << 2) + (x & 3);
+}
+
+__global__ __launch_bounds__(256) void expand_density_kernel(const float* __restrict__ density, int X, int Y, int Z,
+                                                            uint32_t nbx, uint32_t nby, float4* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t nx = (uint64_t)X + 1, ny = (uint64_t)Y + 1, nz = (uint64_t)Z + 1;
+    if (i >= nx * ny * nz) return;
+    const uint32_t x = (uint32_t)(i % nx), y = (uint32_t)((i / nx) % ny), z = (uint32_t)(i / (nx * ny));
+    float v[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        // padded coordinate x + (c & 1) is original coordinate x + (c & 1) - 1
+        const int ox = (int)x + (c & 1) - 1, oy = (int)y + ((c >> 1) & 1) - 1, oz = (int)z + (c >> 2) - 1;
+        const bool in = ox >= 0 && ox < X && oy >= 0 && oy < Y && oz >= 0 && oz < Z;
+        v[c] = in ? density[((size_t)oz * Y + oy) * X + ox] : 0.0f;
+    }
+    const uint64_t r = corner_record(x, y, z, nbx, nby);
+    out[2 * r] = make_float4(v[0], v[1], v[2], v[3]);
+    out[2 * r + 1] = make_float4(v[4], v[5], v[6], v[7]);
+}
+
 // Trilinear density at voxel-unit position (cx, cy, cz) (texel centres at i + 0.5, zero outside
-// the volume) from the zero-bordered, bricked copy (nbx, nby: bricks along x and y).
-__device__ __forceinline__ float density_at(__amdgpu_buffer_rsrc_t rs, int X, int Y, int Z, uint32_t nbx,
+// the volume) from the corner records (nbx, nby: bricks along x and y).
+__device__ __forceinline__ float density_at(const float4* __restrict__ rec, int X, int Y, int Z, uint32_t nbx,
                                             uint32_t nby, float cx, float cy, float cz) {
     const float xb = cx - 0.5f, yb = cy - 0.5f, zb = cz - 0.5f;
     const float fx = floorf(xb), fy = floorf(yb), fz = floorf(zb);
-    // outside: every corner is in the zero border or beyond (and the offsets stay in range)
+    // outside: every corner is in the zero border or beyond (and the record index stays in range)
     if (!(fx > -2.0f && fx < (float)X && fy > -2.0f && fy < (float)Y && fz > -2.0f && fz < (float)Z)) return 0.0f;
     const float ax = xb - fx, ay = yb - fy, az = zb - fz;
-    const uint32_t x = (uint32_t)((int)fx + 1), y = (uint32_t)((int)fy + 1), z = (uint32_t)((int)fz + 1);
-    const uint32_t base = ((((z >> 2) * nby + (y >> 2)) * nbx + (x >> 2)) << 6) + ((z & 3) << 4) + ((y & 3) << 2) + (x & 3);
-    // next voxel along each axis: inside the brick, or the first layer of the neighbouring brick
-    const uint32_t dx = (x & 3) == 3 ? 64 - 3 : 1;
-    const uint32_t dy = (y & 3) == 3 ? (nbx << 6) - 12 : 4;
-    const uint32_t dz = (z & 3) == 3 ? ((nbx * nby) << 6) - 48 : 16;
-    auto ld = [&](uint32_t i) {
-        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)(i << 2), 0, 0));
-    };
-    const float v0 = ld(base), v1 = ld(base + dx), v2 = ld(base + dy), v3 = ld(base + dy + dx);
-    const float v4 = ld(base + dz), v5 = ld(base + dz + dx), v6 = ld(base + dz + dy), v7 = ld(base + dz + dy + dx);
-    const float c00 = fmaf(ax, v1 - v0, v0);
-    const float c10 = fmaf(ax, v3 - v2, v2);
-    const float c01 = fmaf(ax, v5 - v4, v4);
-    const float c11 = fmaf(ax, v7 - v6, v6);
+    const uint64_t r = corner_record((uint32_t)((int)fx + 1), (uint32_t)((int)fy + 1), (uint32_t)((int)fz + 1), nbx, nby);
+    const float4 lo = rec[2 * r], hi = rec[2 * r + 1];
+    const float c00 = fmaf(ax, lo.y - lo.x, lo.x);
+    const float c10 = fmaf(ax, lo.w - lo.z, lo.z);
+    const float c01 = fmaf(ax, hi.y - hi.x, hi.x);
+    const float c11 = fmaf(ax, hi.w - hi.z, hi.z);
     const float c0 = fmaf(ay, c10 - c00, c00);
     const float c1 = fmaf(ay, c11 - c01, c01);
     return fmaf(az, c1 - c0, c0);
@@ -92,12 +111,11 @@ __global__ __launch_bounds__(kTermaSeg * kNpix) void terma_kernel(
     const float fl = fluence[(size_t)b * kNpix + p];
     const bool lit = fl != 0.0f;
 
-    const uint32_t nbx = ((uint32_t)X + 5) / 4, nby = ((uint32_t)Y + 5) / 4, nbz = ((uint32_t)Z + 5) / 4;
-    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)density_pad, (short)0, (int)(nbx * nby * nbz * 64 * sizeof(float)), 0x00020000);
+    const uint32_t nbx = ((uint32_t)X + 4) / 4, nby = ((uint32_t)Y + 4) / 4;
+    const float4* __restrict__ rec = reinterpret_cast<const float4*>(density_pad);
     auto rho_at = [&](int k) {
         const float t = fmaf((float)k, dt, t0);
-        return density_at(rs, X, Y, Z, nbx, nby, fmaf(ax, t, sx), fmaf(ay, t, sy), fmaf(az, t, sz));
+        return density_at(rec, X, Y, Z, nbx, nby, fmaf(ax, t, sx), fmaf(ay, t, sy), fmaf(az, t, sz));
     };
 
     float* __restrict__ T = terma + B.bev_off + p;
@@ -174,6 +192,20 @@ __global__ __launch_bounds__(kTermaSeg * kNpix) void terma_kernel(
 }
 
 }  // namespace
+
+uint64_t density_records_floats(const fd_volume& vol) {
+    const uint64_t nbx = (vol.dim[0] + 4) / 4, nby = (vol.dim[1] + 4) / 4, nbz = (vol.dim[2] + 4) / 4;
+    return nbx * nby * nbz * 64 * 8;
+}
+
+hipError_t launch_expand_density(const float* density, const fd_volume& vol, float* records, hipStream_t stream) {
+    const int X = vol.dim[0], Y = vol.dim[1], Z = vol.dim[2];
+    const uint64_t n = (uint64_t)(X + 1) * (Y + 1) * (Z + 1);
+    const uint32_t nbx = ((uint32_t)X + 4) / 4, nby = ((uint32_t)Y + 4) / 4;
+    hipLaunchKernelGGL(expand_density_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, density, X, Y,
+                       Z, nbx, nby, reinterpret_cast<float4*>(records));
+    return hipGetLastError();
+}
 
 hipError_t launch_terma(const BeamletDev* beams, int n, const float* fluence, const float* density_pad,
                         const fd_volume& vol, const SpectrumDev& spec, float rpl64_len, float* terma,
