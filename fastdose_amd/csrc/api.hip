@@ -288,6 +288,34 @@ int fd_read_kernel(const char* path, int n_phi, fd_kernel* out) {
     return FD_OK;
 }
 
+// Whether the resampler's reciprocal (hardware reciprocal + one Newton step) equals the IEEE
+// quotient for every float with exponent in [-64, 64) on `device` -- the depths a sample can have
+// (calc_range keeps them >= 2^-60).  Checked once per device and process (about 10^9 inputs, a
+// millisecond); where it does not hold the resampler corrects the one significand where the
+// step can tie.
+static hipError_t rcp_exact_on_device(int device, bool& exact) {
+    static std::mutex mu;
+    static int state[256] = {0};  // 0 unknown, 1 exact, 2 not exact
+    std::lock_guard<std::mutex> lock(mu);
+    if (device < 0 || device >= 256) {
+        exact = false;
+        return hipSuccess;
+    }
+    if (state[device] == 0) {
+        uint32_t* d = nullptr;
+        uint32_t h = 1;
+        hipError_t e = hipMalloc((void**)&d, sizeof(uint32_t));
+        if (e == hipSuccess) e = hipMemset(d, 0, sizeof(uint32_t));
+        if (e == hipSuccess) e = launch_rcp_check(-64, 128, d, nullptr);
+        if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof(uint32_t), hipMemcpyDeviceToHost);
+        if (d) hipFree(d);
+        if (e != hipSuccess) return e;
+        state[device] = h == 0 ? 1 : 2;
+    }
+    exact = state[device] == 1;
+    return hipSuccess;
+}
+
 int fd_ctx_create(int device, const fd_volume* vol, const float* density_host, const fd_spectrum* spectrum,
                   const fd_kernel* kernel, fd_ctx** out) {
     if (!vol || !density_host || !spectrum || !kernel || !out) {
@@ -386,6 +414,7 @@ int fd_ctx_create(int device, const fd_volume* vol, const float* density_host, c
             (e = hipDeviceSynchronize()) != hipSuccess)
             return fail(e, "density corner records");
     }
+    if ((e = rcp_exact_on_device(c->device, c->rcp_exact)) != hipSuccess) return fail(e, "reciprocal check");
     if ((e = hipMalloc((void**)&c->spec_tab, tab.size() * sizeof(float))) != hipSuccess)
         return fail(e, "hipMalloc spectrum table");
     if ((e = hipMemcpy(c->spec_tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess)
@@ -425,7 +454,8 @@ int fd_ctx_set_options(fd_ctx* ctx, int options) {
         set_error("null context");
         return FD_ERR_ARG;
     }
-    if (options & ~(FD_OPT_EXACT_G | FD_OPT_RPL_FP64 | FD_OPT_TIMING | FD_OPT_RPL_FP64_ALL | FD_OPT_GRAPH)) {
+    if (options & ~(FD_OPT_EXACT_G | FD_OPT_RPL_FP64 | FD_OPT_TIMING | FD_OPT_RPL_FP64_ALL | FD_OPT_GRAPH |
+                    FD_OPT_IEEE_DIV)) {
         set_error("unknown option bits");
         return FD_ERR_ARG;
     }

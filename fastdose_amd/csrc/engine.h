@@ -146,6 +146,7 @@ struct fd_ctx {
     int options = 0;
     float threshold = 0.0f;  // relative CSR threshold (0: drop exact zeros only)
     int n_cu = 256;          // compute units of the device
+    bool rcp_exact = false;  // the resampler's reciprocal is the IEEE quotient on this device
     fd_tuning tune{8192, 0, 4, 0.0f};  // batched-path launch parameters (fd_autotune)
     fdi::Workspace ws;
     fdi::DevBuf<fdi::FlagBeamDev> flag_desc;  // beamlet-flag kernel inputs / outputs

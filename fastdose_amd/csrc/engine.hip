@@ -248,7 +248,8 @@ int launch_values(const fd_ctx* ctx, Workspace& w, int n, const float* dose, flo
     if (e == hipSuccess)
         e = launch_resample_values(w.desc.ptr, n, ctx->vol, extent, dose, w.cand_off.ptr, cap, w.flags.ptr,
                                    w.tmp_cols.ptr, w.tmp_vals.ptr, w.nnz_count.ptr,
-                                   thr > 0.0f ? w.rowmax.ptr : nullptr, stream);
+                                   thr > 0.0f ? w.rowmax.ptr : nullptr,
+                                   ctx->rcp_exact && !(ctx->options & FD_OPT_IEEE_DIV), stream);
     if (thr > 0.0f) {
         // kept entries per segment into cand_count (free once cand_off is scanned), then offsets
         if (e == hipSuccess)

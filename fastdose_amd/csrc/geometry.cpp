@@ -123,6 +123,10 @@ int calc_range(fd_beam& beam, const fd_volume& vol) {
         set_error("the beam doesn't intersect with the bounding box");
         return FD_ERR_GEOM;
     }
+    if (!(near >= 0x1p-60f)) {  // the BEV depth axis starts at the source; the resampler needs depth > 0
+        set_error("the beam source lies on the bounding box");
+        return FD_ERR_GEOM;
+    }
     beam.lim_min = near;
     beam.lim_max = far;
     beam.long_dim = (uint32_t)std::ceil((beam.lim_max - beam.lim_min) / beam.long_spacing);

@@ -40,10 +40,15 @@ hipError_t launch_resample_count(const BeamletDev* beams, int n, const fd_volume
 // at cand_off[b * Z + z]; nonzero counts into nnz_count.  A segment whose candidates would end
 // beyond `cap` entries writes nothing, gets a zero count and sets overflow[0] = 1.
 // rowmax_bits (nullable, zeroed by the caller): per-beamlet maximum value, as float bits.
+// rcp_exact: launch_rcp_check found no mismatch on this device (see there).
 hipError_t launch_resample_values(const BeamletDev* beams, int n, const fd_volume& vol, float extent,
                                   const float* dose_bev, const uint64_t* cand_off, uint64_t cap, uint32_t* overflow,
                                   uint64_t* tmp_cols, float* tmp_vals, uint32_t* nnz_count, uint32_t* rowmax_bits,
-                                  hipStream_t stream);
+                                  bool rcp_exact, hipStream_t stream);
+// Adds to *mismatches (zeroed by the caller) the number of floats b with exponent in
+// [exp0, exp0 + n_exp) for which the resampler's reciprocal (hardware reciprocal + one Newton
+// step) differs from the IEEE quotient 1 / b.
+hipError_t launch_rcp_check(int exp0, int n_exp, uint32_t* mismatches, hipStream_t stream);
 // Relative threshold: per segment, the entries >= thr * rowmax of its beamlet.
 hipError_t launch_resample_threshold_count(int n, int Z, const uint64_t* cand_off, const uint32_t* nnz_count,
                                            const float* tmp_vals, const uint32_t* rowmax_bits, float thr,

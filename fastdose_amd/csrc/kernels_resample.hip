@@ -133,7 +133,18 @@ __global__ __launch_bounds__(kRowThreads) void resample_count_kernel(const Beaml
     if (threadIdx.x == 0) cand_count[(size_t)b * Z + z] = total;
 }
 
-// value of voxel (x, y, z) for beamlet B; decisions in FP32 with the oracle's order
+// floor(v) as an integer (v_cvt_flr_i32_f32: out-of-range values saturate)
+__device__ __forceinline__ int floor_int(float v) {
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+
+// value of voxel (x, y, z) for beamlet B; decisions in FP32 with the oracle's order.  RCP_EXACT:
+// the device's reciprocal followed by one Newton step equals the IEEE quotient for every input
+// (checked on the device when a context is created); otherwise the one significand where the step
+// can tie is corrected.
+template <bool RCP_EXACT>
 __device__ float voxel_value(const BeamletDev& B, const float* __restrict__ dose, int x, int y, int z, float vsx,
                              float vsy, float vsz, float extent, float half_diag, float half_sdiag) {
     const float* M = B.Rf;
@@ -159,7 +170,6 @@ __device__ float voxel_value(const BeamletDev& B, const float* __restrict__ dose
         if (dist > half_diag + extent) return 0.0f;
     }
     const float inv_ss = 1.0f / kResampleSS;
-    const float Lf = (float)B.L;
     const float half = 0.5f * kFmap;
     const float* __restrict__ D = dose + B.bev_off;
     // per-axis sample coordinates minus source
@@ -172,15 +182,16 @@ __device__ float voxel_value(const BeamletDev& B, const float* __restrict__ dose
         qz[i] = ((float)z + o) * vsz - s2;
     }
     // Projection u = x kx / y + 8 (w alike) with the oracle's rounding: u = fma(x kx, RN(1 / y), 8).
-    // The hardware reciprocal (1 ulp) gives u within a few 1e-6 of that value, so its floor and
-    // range tests agree unless u lies within kNearInt of an integer; only those samples take the
-    // IEEE division.  The rotation's partial products are hoisted out of the sample loops (same
-    // expression, same rounding).
-    constexpr float kNearInt = 4e-5f;
-    // The range tests are the oracle's float compares, combined without branches (the compare
-    // masks are scalar).  The BEV dose is read through a buffer resource of the beamlet's L x 256
-    // values: a sample that fails reads beyond the range and gets 0 from the hardware, so the five
-    // samples of a column issue their loads together and add exactly what the oracle adds.
+    // RN(1 / y) is the hardware reciprocal r (within one ulp) refined by one Newton step with fused
+    // multiply-adds, r + r (1 - y r): correctly rounded whenever r is one of the two floats around
+    // 1 / y, except for the all-ones significand, where the step can tie.  The rotation's partial
+    // products are hoisted out of the sample loops (same expression, same rounding).
+    // The range tests are the oracle's float compares, taken on the floors: for finite values
+    // floor(v) in [0, n) iff 0 <= v < n, and out-of-range floors saturate to values the unsigned
+    // compares reject; dd, u and w are finite whenever dd passes, since then y >= lim_min > 0.
+    // The BEV dose is read through a buffer resource of the beamlet's L x 256 values: a sample that
+    // fails reads beyond the range and gets 0 from the hardware, so the five samples of a column
+    // issue their loads together and add exactly what the oracle adds.
     const uint32_t Lu = (uint32_t)B.L;
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc((void*)D, (short)0, (int)(Lu * kNpix * sizeof(float)), 0x00020000);
@@ -195,24 +206,22 @@ __device__ float voxel_value(const BeamletDev& B, const float* __restrict__ dose
             for (int kk = 0; kk < kResampleSS; kk++) {
                 const float by = fmaf(M[5], qz[kk], ay);
                 const float dd = (by - B.lim_min) * B.inv_ls;
-                const bool okd = (dd >= 0.0f) & (dd < Lf);
                 const float bx = fmaf(M[2], qz[kk], ax);
                 const float bz = fmaf(M[8], qz[kk], az);
                 const float tx = bx * B.kx, tz = bz * B.kz;
-                const float ra = __builtin_amdgcn_rcpf(by);
-                float u = fmaf(tx, ra, half);
-                float w = fmaf(tz, ra, half);
-                // near an integer (|u - n| < kNearInt, w alike): take the IEEE quotient
-                const float nu = __builtin_amdgcn_fractf(u + kNearInt), nw = __builtin_amdgcn_fractf(w + kNearInt);
-                if (fminf(nu, nw) < 2.0f * kNearInt) {
-                    const float rinv = 1.0f / by;
-                    u = fmaf(tx, rinv, half);
-                    w = fmaf(tz, rinv, half);
+                const float r0 = __builtin_amdgcn_rcpf(by);
+                float rinv = fmaf(r0, fmaf(-by, r0, 1.0f), r0);
+                if (!RCP_EXACT) {
+                    // all-ones significand: the quotient's significand is 0x000001, and the step
+                    // gives it or (from below) the tie 0x000000
+                    const uint32_t t = (__float_as_uint(by) & 0x7FFFFFu) + 1u;  // 0x800000 iff all ones
+                    rinv = __uint_as_float(__float_as_uint(rinv) | (t >> 23));
                 }
-                const bool ok = okd & (u >= 0.0f) & (u < (float)kFmap) & (w >= 0.0f) & (w < (float)kFmap);
-                // truncation == floor on the accepted range; the other arm is never selected
-                const uint32_t boff =
-                    ok ? ((uint32_t)(int)dd * kNpix + (uint32_t)(int)w * kFmap + (uint32_t)(int)u) * 4u : kOutside;
+                const float u = fmaf(tx, rinv, half);
+                const float w = fmaf(tz, rinv, half);
+                const int fd = floor_int(dd), fu = floor_int(u), fw = floor_int(w);
+                const bool ok = ((uint32_t)fd < Lu) & ((uint32_t)(fu | fw) < (uint32_t)kFmap);
+                const uint32_t boff = ok ? (((uint32_t)fd * kNpix + (uint32_t)fw * kFmap + (uint32_t)fu) << 2) : kOutside;
                 val[kk] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)boff, 0, 0));
             }
 #pragma unroll
@@ -222,7 +231,10 @@ __device__ float voxel_value(const BeamletDev& B, const float* __restrict__ dose
     return acc * (1.0f / (kResampleSS * kResampleSS * kResampleSS));
 }
 
-__global__ __launch_bounds__(kRowThreads) void resample_values_kernel(
+// At most 64 VGPRs (8 wavefronts per SIMD): the sample gathers need the latency hiding; with the
+// compiler's own choice (126 VGPRs, 4 wavefronts) the kernel runs 25 % slower.
+template <bool RCP_EXACT>
+__global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void resample_values_kernel(
     const BeamletDev* __restrict__ beams, int X, int Y, int Z, float vsx, float vsy, float vsz, float extent,
     const float* __restrict__ dose, const uint64_t* __restrict__ cand_off, uint64_t cap,
     uint32_t* __restrict__ overflow, uint64_t* __restrict__ tmp_cols, float* __restrict__ tmp_vals,
@@ -282,7 +294,7 @@ __global__ __launch_bounds__(kRowThreads) void resample_values_kernel(
             }
             y = lo;
             x = (int)(s_x0[y] + (c - s_off[y]));
-            v = voxel_value(B, dose, x, y, z, vsx, vsy, vsz, extent, half_diag, half_sdiag);
+            v = voxel_value<RCP_EXACT>(B, dose, x, y, z, vsx, vsy, vsz, extent, half_diag, half_sdiag);
         }
         const uint32_t nz = (c < total && v != 0.0f) ? 1u : 0u;
         vmax = fmaxf(vmax, v);
@@ -364,6 +376,17 @@ __global__ __launch_bounds__(256) void resample_compact_kernel(int Z, const uint
     }
 }
 
+// One Newton step from the hardware reciprocal against the IEEE quotient, for every significand
+// and the exponents [exp0, exp0 + gridDim.y); counts the inputs where they differ.
+__global__ __launch_bounds__(256) void rcp_check_kernel(int exp0, uint32_t* __restrict__ mismatches) {
+    const uint32_t m = blockIdx.x * 256u + threadIdx.x;  // the grid covers exactly 2^23 significands
+    const int ex = exp0 + (int)blockIdx.y;
+    const float b = __uint_as_float(((uint32_t)(ex + 127) << 23) | m);
+    const float r0 = __builtin_amdgcn_rcpf(b);
+    const float r = fmaf(r0, fmaf(-b, r0, 1.0f), r0);
+    if (r != 1.0f / b) atomicAdd(mismatches, 1u);
+}
+
 __global__ void row_offsets_kernel(int n, int Z, const uint64_t* __restrict__ nnz_off,
                                    const uint64_t* __restrict__ total, uint64_t* __restrict__ offsets) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -385,13 +408,23 @@ hipError_t launch_resample_count(const BeamletDev* beams, int n, const fd_volume
 hipError_t launch_resample_values(const BeamletDev* beams, int n, const fd_volume& vol, float extent,
                                   const float* dose_bev, const uint64_t* cand_off, uint64_t cap, uint32_t* overflow,
                                   uint64_t* tmp_cols, float* tmp_vals, uint32_t* nnz_count, uint32_t* rowmax_bits,
-                                  hipStream_t stream) {
+                                  bool rcp_exact, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     dim3 grid(vol.dim[2], n), block(kRowThreads);
     const size_t shm = (2 * (size_t)vol.dim[1] + 1) * sizeof(uint32_t);
-    hipLaunchKernelGGL(resample_values_kernel, grid, block, shm, stream, beams, (int)vol.dim[0], (int)vol.dim[1],
-                       (int)vol.dim[2], vol.voxel_size[0], vol.voxel_size[1], vol.voxel_size[2], extent, dose_bev,
-                       cand_off, cap, overflow, tmp_cols, tmp_vals, nnz_count, rowmax_bits);
+    if (rcp_exact)
+        hipLaunchKernelGGL(resample_values_kernel<true>, grid, block, shm, stream, beams, (int)vol.dim[0],
+                           (int)vol.dim[1], (int)vol.dim[2], vol.voxel_size[0], vol.voxel_size[1], vol.voxel_size[2],
+                           extent, dose_bev, cand_off, cap, overflow, tmp_cols, tmp_vals, nnz_count, rowmax_bits);
+    else
+        hipLaunchKernelGGL(resample_values_kernel<false>, grid, block, shm, stream, beams, (int)vol.dim[0],
+                           (int)vol.dim[1], (int)vol.dim[2], vol.voxel_size[0], vol.voxel_size[1], vol.voxel_size[2],
+                           extent, dose_bev, cand_off, cap, overflow, tmp_cols, tmp_vals, nnz_count, rowmax_bits);
+    return hipGetLastError();
+}
+
+hipError_t launch_rcp_check(int exp0, int n_exp, uint32_t* mismatches, hipStream_t stream) {
+    hipLaunchKernelGGL(rcp_check_kernel, dim3((1u << 23) / 256, n_exp), dim3(256), 0, stream, exp0, mismatches);
     return hipGetLastError();
 }
 

@@ -48,8 +48,12 @@ enum {
                                 than 256 voxels (long_dim * long_spacing > 256 * min voxel size) */
     FD_OPT_TIMING = 4,       /* record HIP events around every kernel launch (fd_ctx_timings) */
     FD_OPT_RPL_FP64_ALL = 8, /* FP64 radiological path on every ray */
-    FD_OPT_GRAPH = 16        /* batched paths (fd_dosemat_add_beam(s), serving) replay each batch as one
+    FD_OPT_GRAPH = 16,       /* batched paths (fd_dosemat_add_beam(s), serving) replay each batch as one
                                 captured HIP graph instead of separate launches */
+    FD_OPT_IEEE_DIV = 32     /* resampler: correct the refined hardware reciprocal on depths whose
+                                significand is all ones even where the context's device check found
+                                it exact there (results are identical either way; no counterpart
+                                in the reference) */
 };
 
 /* Value types of packed dose-loading matrices (fd_dosemat_pack, fd_spmm_create). */

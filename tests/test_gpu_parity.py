@@ -235,6 +235,24 @@ def test_batched_beams_match_per_beam(torch_cuda):
         assert ref.matrix(2)[0].tolist() == [0]
 
 
+def test_reciprocal_variants(torch_cuda):
+    """The resampler with the significand correction forced on (FD_OPT_IEEE_DIV) gives the same
+    matrix, bit for bit, as the default (which skips it where the device check passed), and both
+    pass the oracle checks."""
+    cfg = synth.small_config(seed=23, dim=(24, 26, 22), fluence_dim=2, angles=(-35.0, 25.0, 60.0))
+    beams, flu = beamlets_of(cfg)
+    out = []
+    for opt in (0, fd.FD_OPT_IEEE_DIV):
+        log(f"reciprocal variant options={opt}")
+        with fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel, options=opt) as ctx:
+            out.append(gpu_stages(ctx, beams, flu, cfg.extent))
+    for i in range(len(beams)):
+        c0, v0 = row(out[0], i)
+        c1, v1 = row(out[1], i)
+        assert np.array_equal(c0, c1) and np.array_equal(v0, v1), i
+    check_against_oracle(cfg, beams, flu, out[1], range(len(beams)))
+
+
 @pytest.mark.parametrize("segments", [1, 2, 4])
 def test_terma_segments(torch_cuda, segments):
     """Terma with 1, 2 or 4 ray segments per workgroup (a tuning parameter) stays within the bar."""
