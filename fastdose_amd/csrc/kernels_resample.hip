@@ -11,8 +11,8 @@
 // the dense nBeamlets x nVoxels intermediate of the reference never exists.
 //
 // Work decomposition.  One workgroup per (z-plane, beamlet).  For each row y of the plane the
-// workgroup bounds the x-range of voxels whose box can reach the beamlet's frustum (six linear
-// inequalities solved in FP64 with a margin); the concatenation of those ranges, in (y, x)
+// workgroup bounds the x-range of voxels whose supersamples can reach the beamlet's frustum (six
+// linear inequalities solved in FP64 with a margin); the concatenation of those ranges, in (y, x)
 // order, is the candidate list.  Stage 1 counts candidates, stage 2 evaluates them and compacts
 // nonzeros in voxel order, stage 3 copies them into the CSR arrays.  All decisions (culling
 // tests, which BEV voxel a supersample hits) use FP32 with the operation order of the CPU oracle.
@@ -42,8 +42,10 @@ __device__ __forceinline__ void plane_row(const BeamletDev& B, int y, int z, flo
     }
 }
 
-// Candidate x-range [x0, x1) of row (y, z).  rho bounds the distance of any point of a voxel box
-// from its centre plus a rounding margin.
+// Candidate x-range [x0, x1) of row (y, z): the voxels whose supersamples can fall inside the
+// beamlet's BEV frustum (depth in [lim_min, lim_min + L ls], |x|, |z| <= half the 16-pixel field
+// at that depth).  rho bounds the distance of any supersample from its voxel centre plus a
+// rounding margin.
 __device__ __forceinline__ void row_interval(const BeamletDev& B, int y, int z, int X, float vsx, float vsy,
                                              float vsz, double rho, int& x0, int& x1) {
     PlaneGeom pg;
@@ -77,15 +79,18 @@ __device__ __forceinline__ void row_interval(const BeamletDev& B, int y, int z, 
         x1 = 0;
         return;
     }
-    int a = (int)floor(lo) - 1, c = (int)floor(hi) + 2;
+    // the voxels whose centre lies in [lo, hi] (the margin is in rho)
+    int a = (int)ceil(lo), c = (int)floor(hi) + 1;
     a = a < 0 ? 0 : a;
     c = c > X ? X : c;
     x0 = a;
     x1 = c > a ? c : a;
 }
 
+// Largest distance of a supersample from its voxel centre, (1/2 - 1/(2 n)) |voxel diagonal| for
+// n samples per axis, plus 0.1 mm for the rounding of the FP32 sample decisions.
 __device__ __forceinline__ double rho_of(float vsx, float vsy, float vsz) {
-    return 0.5 * sqrt((double)vsx * vsx + (double)vsy * vsy + (double)vsz * vsz) + 1e-2;
+    return (0.5 - 0.5 / kResampleSS) * sqrt((double)vsx * vsx + (double)vsy * vsy + (double)vsz * vsz) + 1e-2;
 }
 
 // block-wide exclusive scan of one u32 per thread (blockDim.x <= 1024, multiple of 64)
