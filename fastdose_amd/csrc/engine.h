@@ -87,6 +87,11 @@ struct Workspace {
     size_t flu_cap = 0;
     hipEvent_t stage_done = nullptr;
     DevBuf<BeamletDev> desc;
+    // what desc holds after the last upload_beams (valid until a batch overwrites it), and the
+    // stream that upload was ordered on
+    std::vector<BeamletDev> desc_host, desc_next;
+    bool desc_valid = false;
+    hipStream_t desc_stream = nullptr;
     DevBuf<float> partial;
     DevBuf<float> fluence, terma, dens_bev, dose_bev;
     DevBuf<uint32_t> cand_count, nnz_count;
@@ -184,10 +189,11 @@ int run_terma(const fd_ctx* ctx, Workspace& w, int options, int n, const float* 
               uint64_t stride, hipStream_t stream);
 int run_cccs(const fd_ctx* ctx, Workspace& w, int options, int n, int max_L, const float* terma, const float* dens,
              float* dose, uint64_t stride, hipStream_t stream);
-// Resampler stages 1-2 with exact sizing (two host synchronisations); leaves the offsets in w.
-// thr > 0 keeps only entries >= thr * (the beamlet's maximum value).
-int run_resample_values(const fd_ctx* ctx, Workspace& w, int n, const float* dose, float extent, float thr,
-                        uint64_t* nnz_total, hipStream_t stream);
+// Resampler stages 1-2 (one host synchronisation; a second only when the estimated staging
+// overflows); leaves the offsets in w.  thr > 0 keeps only entries >= thr * (the beamlet's
+// maximum value).
+int run_resample_values(const fd_ctx* ctx, Workspace& w, const fd_beam* beams, int n, const float* dose,
+                        float extent, float thr, uint64_t* nnz_total, hipStream_t stream);
 int run_resample_compact(const fd_ctx* ctx, Workspace& w, int n, float thr, uint64_t* offsets, uint64_t* columns,
                          float* values, hipStream_t stream);
 

@@ -98,6 +98,20 @@ namespace {
 
 // Pinned staging of the descriptors (and optionally the fluence) of a batch; waits until the
 // previous upload from this workspace has left the staging area.
+// Device descriptors of n beamlets with BEV volumes `stride` floats apart.  Beamlets of one beam
+// share the orientation, so the rotation matrices are computed once per run of equal angles.
+void build_descriptors(const fd_beam* beams, int n, uint64_t stride, BeamletDev* out) {
+    for (int i = 0; i < n; i++) {
+        if (i > 0 && std::memcmp(beams[i].angles, beams[i - 1].angles, sizeof(beams[i].angles)) == 0) {
+            std::memcpy(out[i].Rf, out[i - 1].Rf, sizeof(out[i].Rf));
+            std::memcpy(out[i].Ri, out[i - 1].Ri, sizeof(out[i].Ri));
+            fill_beamlet_dev_no_rotation(beams[i], (uint64_t)i * stride, out[i]);
+        } else {
+            fill_beamlet_dev(beams[i], (uint64_t)i * stride, out[i]);
+        }
+    }
+}
+
 int stage_inputs(Workspace& w, const fd_beam* beams, int n, uint64_t stride, const float* fluence_host) {
     FD_HIP(hipEventSynchronize(w.stage_done));
     if ((size_t)n > w.stage_cap) {
@@ -107,7 +121,7 @@ int stage_inputs(Workspace& w, const fd_beam* beams, int n, uint64_t stride, con
         FD_HIP(hipHostMalloc((void**)&w.stage, (size_t)n * sizeof(BeamletDev), hipHostMallocDefault));
         w.stage_cap = n;
     }
-    for (int i = 0; i < n; i++) fill_beamlet_dev(beams[i], (uint64_t)i * stride, w.stage[i]);
+    if (beams) build_descriptors(beams, n, stride, w.stage);
     if (fluence_host) {
         const size_t nf = (size_t)n * kNpix;
         if (nf > w.flu_cap) {
@@ -138,6 +152,12 @@ double frustum_voxels(const fd_beam& b, const fd_volume& vol) {
     return std::max(F(z1) - F(z0), 0.0) / (vx * vy * vz);
 }
 
+// Follow the observed candidates per frustum voxel, with headroom; shrink slowly.
+void update_candidate_estimate(Workspace& w, double ratio) {
+    w.cand_per_beamlet = std::max(1.15 * ratio, 0.95 * w.cand_per_beamlet);
+    if (w.cand_per_beamlet > 1.5 * 1.15 * ratio) w.cand_per_beamlet = 1.5 * 1.15 * ratio;
+}
+
 }  // namespace
 
 size_t batch_footprint(const fd_ctx* ctx, const fd_beam* beams, int n) {
@@ -152,10 +172,25 @@ size_t batch_footprint(const fd_ctx* ctx, const fd_beam* beams, int n) {
 
 int upload_beams(Workspace& w, const fd_beam* beams, int n, uint64_t stride, hipStream_t stream) {
     if (n == 0) return FD_OK;
-    FD_TRY(stage_inputs(w, beams, n, stride, nullptr));
+    // The stages of one beam (fd_terma, fd_cccs, fd_resample_csr) pass the same beamlets: when the
+    // device already holds these exact descriptors, ordered on this stream or already copied, the
+    // upload (and the copy engine's hand-off between two kernels) is skipped.
+    w.desc_next.resize((size_t)n);
+    build_descriptors(beams, n, stride, w.desc_next.data());
+    const size_t bytes = (size_t)n * sizeof(BeamletDev);
+    if (w.desc_valid && w.desc_host.size() == (size_t)n &&
+        std::memcmp(w.desc_host.data(), w.desc_next.data(), bytes) == 0 &&
+        (w.desc_stream == stream || hipEventQuery(w.stage_done) == hipSuccess))
+        return FD_OK;
+    w.desc_valid = false;
+    FD_TRY(stage_inputs(w, nullptr, n, stride, nullptr));
+    std::memcpy(w.stage, w.desc_next.data(), bytes);
     FD_TRY(w.desc.reserve(n));
-    FD_HIP(hipMemcpyAsync(w.desc.ptr, w.stage, (size_t)n * sizeof(BeamletDev), hipMemcpyHostToDevice, stream));
+    FD_HIP(hipMemcpyAsync(w.desc.ptr, w.stage, bytes, hipMemcpyHostToDevice, stream));
     FD_HIP(hipEventRecord(w.stage_done, stream));
+    w.desc_host.swap(w.desc_next);
+    w.desc_valid = true;
+    w.desc_stream = stream;
     return FD_OK;
 }
 
@@ -293,20 +328,35 @@ int fetch_totals(const fd_ctx* ctx, Workspace& w, int n, hipStream_t stream, boo
 
 }  // namespace
 
-int run_resample_values(const fd_ctx* ctx, Workspace& w, int n, const float* dose, float extent, float thr,
-                        uint64_t* nnz_total, hipStream_t stream) {
-    const size_t segs = (size_t)n * ctx->vol.dim[2];
+int run_resample_values(const fd_ctx* ctx, Workspace& w, const fd_beam* beams, int n, const float* dose, float extent,
+                        float thr, uint64_t* nnz_total, hipStream_t stream) {
     FD_TRY(reserve_segments(ctx, w, n));
+    // staging sized from the running candidate estimate (no host round trip before the values
+    // pass); the values pass refuses overflowing segments and is re-run once with the exact size
+    double est = 0;
+    for (int i = 0; i < n; i++) est += frustum_voxels(beams[i], ctx->vol);
+    if (w.cand_per_beamlet <= 0) w.cand_per_beamlet = 2.0;
+    uint64_t cap = std::max<uint64_t>((uint64_t)std::ceil(w.cand_per_beamlet * est), 1024);
+    FD_TRY(w.tmp_cols.reserve(cap));
+    FD_TRY(w.tmp_vals.reserve(cap));
     FD_TRY(launch_count(ctx, w, n, stream));
-    FD_HIP(hipMemcpyAsync(w.host_totals, w.cand_off.ptr + segs, sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+    FD_TRY(launch_values(ctx, w, n, dose, extent, thr, cap, stream));
+    FD_TRY(fetch_totals(ctx, w, n, stream, false));
     FD_HIP(hipStreamSynchronize(stream));
     const uint64_t ncand = w.host_totals[0];
-    FD_TRY(w.tmp_cols.reserve(std::max<uint64_t>(ncand, 1)));
-    FD_TRY(w.tmp_vals.reserve(std::max<uint64_t>(ncand, 1)));
-    FD_TRY(launch_values(ctx, w, n, dose, extent, thr, ncand, stream));
-    FD_HIP(hipMemcpyAsync(w.host_totals + 1, w.nnz_off.ptr + segs, sizeof(uint64_t), hipMemcpyDeviceToHost,
-                          stream));
-    FD_HIP(hipStreamSynchronize(stream));
+    if (w.host_totals[2]) {
+        cap = ncand;
+        FD_TRY(w.tmp_cols.reserve(cap));
+        FD_TRY(w.tmp_vals.reserve(cap));
+        FD_TRY(launch_values(ctx, w, n, dose, extent, thr, cap, stream));
+        FD_TRY(fetch_totals(ctx, w, n, stream, false));
+        FD_HIP(hipStreamSynchronize(stream));
+        if (w.host_totals[2]) {
+            set_error("resampler staging overflow with exact sizing");
+            return FD_ERR_HIP;
+        }
+    }
+    update_candidate_estimate(w, est > 0 ? (double)ncand / est : 1.0);
     *nnz_total = w.host_totals[1];
     return FD_OK;
 }
@@ -337,6 +387,7 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
     if (w.cand_per_beamlet <= 0) w.cand_per_beamlet = 2.0;  // candidates per frustum voxel, first guess
     uint64_t cap = std::max<uint64_t>((uint64_t)std::ceil(w.cand_per_beamlet * est), 1024);
 
+    w.desc_valid = false;  // the batch's descriptors replace the single-stage ones
     FD_TRY(w.desc.reserve(n));
     FD_TRY(w.fluence.reserve((size_t)n * kNpix));
     FD_TRY(w.terma.reserve((size_t)n * stride));
@@ -422,9 +473,7 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
             return FD_ERR_HIP;
         }
     }
-    // follow the observed candidate density, with headroom; shrink slowly
-    w.cand_per_beamlet = std::max(1.15 * ratio, 0.95 * w.cand_per_beamlet);
-    if (w.cand_per_beamlet > 1.5 * 1.15 * ratio) w.cand_per_beamlet = 1.5 * 1.15 * ratio;
+    update_candidate_estimate(w, ratio);
     out->nnz = w.host_totals[1];
     out->rowoff.assign(w.host_rowoff, w.host_rowoff + n + 1);
     return FD_OK;

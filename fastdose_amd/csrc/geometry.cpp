@@ -365,6 +365,10 @@ int ptv_isocenter(const uint8_t* mask, const fd_volume& vol, float iso[3]) {
 void fill_beamlet_dev(const fd_beam& b, uint64_t bev_off, BeamletDev& d) {
     beam_matrix(b.angles[0], b.angles[1], b.angles[2], false, d.Rf);
     beam_matrix(b.angles[0], b.angles[1], b.angles[2], true, d.Ri);
+    fill_beamlet_dev_no_rotation(b, bev_off, d);
+}
+
+void fill_beamlet_dev_no_rotation(const fd_beam& b, uint64_t bev_off, BeamletDev& d) {
     for (int i = 0; i < 3; i++) d.src[i] = b.source[i];
     d.bs[0] = b.beamlet_size[0];
     d.bs[1] = b.beamlet_size[1];

@@ -18,5 +18,7 @@ int flag_beam_setup(const fd_bundle& bb, const fd_volume& vol, FlagBeamDev& d);
 int ptv_isocenter(const uint8_t* mask, const fd_volume& vol, float iso[3]);
 
 void fill_beamlet_dev(const fd_beam& b, uint64_t bev_off, BeamletDev& d);
+// Everything but the rotation matrices Rf / Ri.
+void fill_beamlet_dev_no_rotation(const fd_beam& b, uint64_t bev_off, BeamletDev& d);
 
 }  // namespace fdi
