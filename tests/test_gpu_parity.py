@@ -153,6 +153,29 @@ def test_deterministic_and_batch_independent(torch_cuda):
         assert np.array_equal(g1[key], g2[key]), key
 
 
+def test_stage_descriptors_follow_the_beams(torch_cuda):
+    """The stage calls reuse the device's beamlet descriptors only when they are the ones passed:
+    stages of beams A, of beams B (same count), a batched run (which replaces the descriptors) and
+    A again give A's first result bit for bit, and B equals B in a fresh context."""
+    cfg = synth.small_config(seed=31, dim=(24, 26, 22), fluence_dim=2)
+    iso = cfg.bundles[0].isocenter
+    cfg.bundles.append(synth.make_bundle(iso, (130.0, -20.0, 10.0), fluence_dim=2))
+    cfg.flags.append(np.ones(4, dtype=np.uint8))
+    a, fa = fd.beamlets_init(cfg.bundles[0], cfg.flags[0], cfg.volume)
+    b, fb = fd.beamlets_init(cfg.bundles[1], cfg.flags[1], cfg.volume)
+    assert len(a) == len(b)
+    with fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel) as ctx:
+        ga = gpu_stages(ctx, a, fa, cfg.extent)
+        gb = gpu_stages(ctx, b, fb, cfg.extent)
+        fd.dose_mat_construction(ctx, cfg.bundles, cfg.flags, cfg.extent, batched=True)
+        ga2 = gpu_stages(ctx, a, fa, cfg.extent)
+    with fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel) as ctx:
+        gb_fresh = gpu_stages(ctx, b, fb, cfg.extent)
+    for key in ("terma", "dose", "cols", "vals", "offsets"):
+        assert np.array_equal(ga[key], ga2[key]), key
+        assert np.array_equal(gb[key], gb_fresh[key]), key
+
+
 def test_dense_to_csr_known_and_random(torch_cuda):
     torch = torch_cuda
     dense = np.array([[0, 0, 1, 0, 2, 0], [2, 0, 0, 1, 0, 0], [0, 0, 1, 2, 0, 1], [1, 0, 1, 0, 0, 0],
