@@ -14,14 +14,16 @@
 // share cache lines.
 //
 // The radiological path is a prefix sum along each ray, which one lane per ray would walk alone
-// (256 rays per beamlet: one wavefront per SIMD on small batches).  Each ray is cut into 1, 2 or
-// 4 segments of slices (a launch parameter), one wavefront group per segment: pass 1 writes the
+// (256 rays per beamlet: one wavefront per SIMD on small batches).  Each ray is cut into 1, 2, 4
+// or 8 segments of slices (a launch parameter; with 8, a beamlet's rays take two workgroups of
+// 128), one wavefront group per segment: pass 1 writes the
 // mean density of its slices, sums rho |step| over its segment and (FP32 path) stores the running
 // in-segment sums of the four samples of each slice in a scratch buffer; after a barrier each
 // segment adds the sum of the segments before it (in order) and pass 2 reads the stored sums back
 // for the Terma.  (The FP64 path walks its samples again instead of storing doubles.)  Segment 0
 // reproduces the sequential accumulation exactly; later segments differ from it by the rounding of
 // one regrouped sum.  Rays with zero fluence skip pass 2's spectrum sum: their Terma is exactly 0.
+#include <algorithm>
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -30,7 +32,8 @@ namespace fdi {
 
 namespace {
 
-constexpr int kTermaSeg = 4;  // most ray segments per beamlet (wavefront groups of the workgroup)
+constexpr int kTermaThreads = 1024;  // workgroup size limit: ray segments x rays per workgroup
+constexpr int kTermaMaxSeg = 8;     // most ray segments (8: two workgroups of 128 rays per beamlet)
 
 // Corner records: for every base voxel (x, y, z) in [0, X] x [0, Y] x [0, Z] of the density with a
 // one-voxel zero border, its eight trilinear corners (x..x+1, y..y+1, z..z+1; x fastest) as two
@@ -81,16 +84,19 @@ __device__ __forceinline__ float density_at(const float4* __restrict__ rec, int 
 
 // rpl64_len: beamlets whose central-axis path (long_dim * long_spacing) exceeds this many cm
 // accumulate the radiological path in FP64 (0: every beamlet; infinity: none).
-__global__ __launch_bounds__(kTermaSeg * kNpix) void terma_kernel(
+// Workgroup (beamlet blockIdx.x, ray block blockIdx.y): `rays` consecutive BEV pixels (a multiple
+// of 64) times blockDim.x / rays segments.
+__global__ __launch_bounds__(kTermaThreads) void terma_kernel(
     const BeamletDev* __restrict__ beams, const float* __restrict__ fluence, const float* __restrict__ density_pad,
     int X, int Y, int Z, float vsx, float vsy, float vsz, SpectrumDev spec, float rpl64_len,
-    float* __restrict__ terma, float* __restrict__ dens_bev, float4* __restrict__ scratch) {
-    __shared__ float s_seg[kTermaSeg][kNpix];
-    __shared__ double s_seg64[kTermaSeg][kNpix];
+    float* __restrict__ terma, float* __restrict__ dens_bev, float4* __restrict__ scratch, int rays) {
+    __shared__ float s_seg[kTermaThreads];
+    __shared__ double s_seg64[kTermaThreads];
     const int b = blockIdx.x;
-    const int p = threadIdx.x & (kNpix - 1);
-    const int seg = __builtin_amdgcn_readfirstlane(threadIdx.x / kNpix);
-    const int nseg = blockDim.x / kNpix;
+    const int lr = threadIdx.x % rays;
+    const int p = blockIdx.y * rays + lr;
+    const int seg = __builtin_amdgcn_readfirstlane(threadIdx.x / rays);
+    const int nseg = blockDim.x / rays;
     const BeamletDev& B = beams[b];
     const int ix = p & (kFmap - 1), iy = p >> 4;
     const float half = 0.5f * kFmap;
@@ -125,8 +131,8 @@ __global__ __launch_bounds__(kTermaSeg * kNpix) void terma_kernel(
     const int i0 = (seg * B.L) / nseg, i1 = ((seg + 1) * B.L) / nseg;
 
     // pass 1: mean density per slice, radiological length of the segment
-    float lr = 0.0f;
-    double lr64 = 0.0;
+    float lpath = 0.0f;
+    double lpath64 = 0.0;
     for (int i = i0; i < i1; i++) {
         float dsum = 0.0f;
         float q[kSuperSampling];
@@ -134,23 +140,23 @@ __global__ __launch_bounds__(kTermaSeg * kNpix) void terma_kernel(
         for (int j = 0; j < kSuperSampling; j++) {
             const float rho = rho_at(kSuperSampling * i + j + 1);
             if (RPL64)
-                lr64 += (double)rho * (double)step_len;
+                lpath64 += (double)rho * (double)step_len;
             else
-                lr = fmaf(rho, step_len, lr);
-            q[j] = lr;
+                lpath = fmaf(rho, step_len, lpath);
+            q[j] = lpath;
             dsum += rho;
         }
         R[(size_t)i * kNpix] = dsum * (1.0f / kSuperSampling);
         if (!RPL64 && lit) Q[(size_t)i * kNpix] = make_float4(q[0], q[1], q[2], q[3]);
     }
-    s_seg[seg][p] = lr;
-    s_seg64[seg][p] = lr64;
+    s_seg[threadIdx.x] = lpath;
+    s_seg64[threadIdx.x] = lpath64;
     __syncthreads();
     float rpl = 0.0f;
     double rpl64 = 0.0;
     for (int s2 = 0; s2 < seg; s2++) {
-        rpl += s_seg[s2][p];
-        rpl64 += s_seg64[s2][p];
+        rpl += s_seg[s2 * rays + lr];
+        rpl64 += s_seg64[s2 * rays + lr];
     }
 
     // pass 2: Terma
@@ -211,12 +217,13 @@ hipError_t launch_terma(const BeamletDev* beams, int n, const float* fluence, co
                         const fd_volume& vol, const SpectrumDev& spec, float rpl64_len, float* terma,
                         float* dens_bev, float* scratch, int segments, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    if (segments != 1 && segments != 2 && segments != kTermaSeg) return hipErrorInvalidValue;
-    dim3 grid(n), block(segments * kNpix);
+    if (segments != 1 && segments != 2 && segments != 4 && segments != kTermaMaxSeg) return hipErrorInvalidValue;
+    const int rays = std::min(kNpix, kTermaThreads / segments);
+    dim3 grid(n, kNpix / rays), block(segments * rays);
     const int X = vol.dim[0], Y = vol.dim[1], Z = vol.dim[2];
     hipLaunchKernelGGL(terma_kernel, grid, block, 0, stream, beams, fluence, density_pad, X, Y, Z, vol.voxel_size[0],
                        vol.voxel_size[1], vol.voxel_size[2], spec, rpl64_len, terma, dens_bev,
-                       reinterpret_cast<float4*>(scratch));
+                       reinterpret_cast<float4*>(scratch), rays);
     return hipGetLastError();
 }
 
