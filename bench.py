@@ -68,6 +68,18 @@ def cpu_baseline(cfg, beams, flu, sample: int, gpu_rows=None, n_check: int = 4):
                                orc.cast(orc.Spectrum, cfg.spectrum), orc.cast(orc.Kernel, cfg.kernel), cfg.extent, 0,
                                threads)
     dt = time.perf_counter() - t0
+    # one-core figure on the first few beamlets of the same sample
+    n1 = min(4, sample)
+    t1 = time.perf_counter()
+    orc.dose_beamlets(sub[:n1], flu[0], orc.cast(orc.Volume, cfg.volume), cfg.density,
+                      orc.cast(orc.Spectrum, cfg.spectrum), orc.cast(orc.Kernel, cfg.kernel), cfg.extent, 0, 1)
+    dt1 = time.perf_counter() - t1
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     out = {
         "value": round(sample / dt, 3),
         "unit": "beamlets/s",
@@ -75,6 +87,8 @@ def cpu_baseline(cfg, beams, flu, sample: int, gpu_rows=None, n_check: int = 4):
         "kind": "port",
         "sample": f"{sample} of the beam's {len(beams)} beamlets (every {len(beams) // sample}th), full FP64 path "
                   f"Terma->CCCS->resample, OpenMP over beamlets, {dt:.1f} s wall",
+        "one_core": {"value": round(n1 / dt1, 3), "beamlets": n1, "wall_s": round(dt1, 2)},
+        "host": {"cpu_model": model, "nproc": os.cpu_count()},
     }
     if gpu_rows is not None:
         offsets, cols, vals = gpu_rows
