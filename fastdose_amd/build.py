@@ -27,9 +27,10 @@ CLI = PKG / "fastdose_imrt"
 ORACLE_DIR = ROOT / "oracle"
 
 # Per-file code generation options.  kernels_cccs.hip: coalescing the copies on split edges keeps
-# more of the step loop's X states in one register set across the rotation branches (CCCS kernel
-# 1.52 -> 1.48 ms on the 128^3 slab beam).
-EXTRA_FLAGS = {"kernels_cccs.hip": ["-mllvm", "-join-splitedges=true"]}
+# more of the step loop's X states in one register set across the rotation branches, and the ILP
+# scheduling strategy interleaves the independent per-slot chains better (CCCS kernel 1.52 -> 1.48
+# -> 1.46 ms on the 128^3 slab beam).
+EXTRA_FLAGS = {"kernels_cccs.hip": ["-mllvm", "-join-splitedges=true", "-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
 
 HIP_SOURCES = ["api.hip", "engine.hip", "serve.hip", "kernels_terma.hip", "kernels_cccs.hip", "kernels_resample.hip", "kernels_csr.hip", "kernels_flags.hip", "kernels_spmm.hip", "spmm.hip"]
 CXX_SOURCES = ["geometry.cpp"]
