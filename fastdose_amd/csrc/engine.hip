@@ -84,6 +84,10 @@ int check_beams(const fd_beam* beams, int n) {
             set_error("beamlet sad, long_spacing and beamlet_size must be positive");
             return FD_ERR_ARG;
         }
+        if (!(beams[i].lim_min >= 0x1p-60f)) {  // as fd_calc_range guarantees; the resampler needs depth > 0
+            set_error("beamlet lim_min must be positive (the source lies outside the density bounding box)");
+            return FD_ERR_ARG;
+        }
     }
     return FD_OK;
 }

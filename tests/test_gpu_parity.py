@@ -389,6 +389,35 @@ def test_graph_replays_match_eager(torch_cuda):
             assert all(np.array_equal(a, b) for a, b in zip(ref.matrix(k), dm.matrix(k))), ("serve", k)
 
 
+def test_empty_and_dark_inputs(torch_cuda):
+    """No beamlets: every stage is a no-op and the CSR has one zero offset.  A beamlet with zero
+    fluence: zero Terma, zero dose, an empty CSR row between two ordinary ones (exact zeros are
+    dropped, as cusparseDenseToSparse does), and the neighbours equal their results alone."""
+    torch = torch_cuda
+    cfg = synth.small_config(seed=13, dim=(20, 22, 18), fluence_dim=2)
+    beams, flu = beamlets_of(cfg)
+    assert len(beams) >= 3
+    with fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel) as ctx:
+        T = torch.zeros(256, device="cuda")
+        ctx.terma([], torch.zeros(256, device="cuda"), T, T, 256)
+        ctx.cccs([], T, T, T, 256)
+        off, cols, vals = ctx.resample_csr([], T, 256, cfg.extent)
+        torch.cuda.synchronize()
+        assert off.cpu().tolist() == [0]
+        dark = flu.copy()
+        dark[1] = 0.0
+        g = gpu_stages(ctx, beams, dark, cfg.extent)
+        ref = gpu_stages(ctx, beams, flu, cfg.extent)
+    L = beams[1].long_dim
+    assert not g["terma"][1, :L].any() and not g["dose"][1, :L].any()
+    c1, v1 = row(g, 1)
+    assert len(c1) == 0
+    for k in (0, 2):
+        ca, va = row(g, k)
+        cb, vb = row(ref, k)
+        assert np.array_equal(ca, cb) and np.array_equal(va, vb), k
+
+
 def test_error_paths(torch_cuda):
     torch = torch_cuda
     cfg = synth.small_config(seed=11, dim=(16, 16, 16), fluence_dim=1)
