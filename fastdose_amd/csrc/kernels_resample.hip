@@ -120,22 +120,28 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
     return res;
 }
 
+// Candidates per (beamlet, z-plane): a workgroup takes kCountPlanes planes of one beamlet, its
+// threads spread over all (plane, row) pairs; integer sums, so the order does not matter.
+constexpr int kCountPlanes = 8;
+
 __global__ __launch_bounds__(kRowThreads) void resample_count_kernel(const BeamletDev* __restrict__ beams, int X,
                                                                      int Y, int Z, float vsx, float vsy, float vsz,
                                                                      uint32_t* __restrict__ cand_count) {
-    __shared__ uint32_t s_wave[kRowThreads / 64 + 1];
-    const int z = blockIdx.x, b = blockIdx.y;
+    __shared__ uint32_t s_cnt[kCountPlanes];
+    const int z0 = blockIdx.x * kCountPlanes, b = blockIdx.y;
+    const int np = min(kCountPlanes, Z - z0);
     const BeamletDev& B = beams[b];
     const double rho = rho_of(vsx, vsy, vsz);
-    uint32_t cnt = 0;
-    for (int y = threadIdx.x; y < Y; y += blockDim.x) {
+    if (threadIdx.x < kCountPlanes) s_cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    for (int i = threadIdx.x; i < np * Y; i += blockDim.x) {
+        const int p = i / Y, y = i - p * Y;
         int x0, x1;
-        row_interval(B, y, z, X, vsx, vsy, vsz, rho, x0, x1);
-        cnt += (uint32_t)(x1 - x0);
+        row_interval(B, y, z0 + p, X, vsx, vsy, vsz, rho, x0, x1);
+        if (x1 > x0) atomicAdd(&s_cnt[p], (uint32_t)(x1 - x0));
     }
-    uint32_t total;
-    block_exclusive_scan(cnt, s_wave, total);
-    if (threadIdx.x == 0) cand_count[(size_t)b * Z + z] = total;
+    __syncthreads();
+    if (threadIdx.x < np) cand_count[(size_t)b * Z + z0 + threadIdx.x] = s_cnt[threadIdx.x];
 }
 
 // floor(v) as an integer (v_cvt_flr_i32_f32: out-of-range values saturate)
@@ -404,7 +410,7 @@ __global__ void row_offsets_kernel(int n, int Z, const uint64_t* __restrict__ nn
 hipError_t launch_resample_count(const BeamletDev* beams, int n, const fd_volume& vol, uint32_t* cand_count,
                                  hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    dim3 grid(vol.dim[2], n), block(kRowThreads);
+    dim3 grid((vol.dim[2] + kCountPlanes - 1) / kCountPlanes, n), block(kRowThreads);
     hipLaunchKernelGGL(resample_count_kernel, grid, block, 0, stream, beams, (int)vol.dim[0], (int)vol.dim[1],
                        (int)vol.dim[2], vol.voxel_size[0], vol.voxel_size[1], vol.voxel_size[2], cand_count);
     return hipGetLastError();
