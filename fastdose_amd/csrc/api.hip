@@ -558,13 +558,13 @@ int fd_resample_csr(fd_ctx* ctx, const fd_beam* beams, int n, const float* dose_
     }
     FD_TRY(upload_beams(ctx->ws, beams, n, stride, s));
     uint64_t nnz = 0;
-    FD_TRY(run_resample_values(ctx, ctx->ws, beams, n, dose_bev_dev, extent, ctx->threshold, &nnz, s));
+    FD_TRY(run_resample_csr(ctx, ctx->ws, beams, n, dose_bev_dev, extent, ctx->threshold, *out, &nnz, s));
     out->nnz = nnz;
     if (nnz > out->capacity || (nnz > 0 && (!out->columns || !out->values))) {
         set_error("CSR capacity too small; nnz reported in out->nnz");
         return FD_ERR_CAPACITY;
     }
-    return run_resample_compact(ctx, ctx->ws, n, ctx->threshold, out->offsets, out->columns, out->values, s);
+    return FD_OK;
 }
 
 int fd_dense_to_csr(const float* dense_dev, uint64_t rows, uint64_t cols, fd_csr* out, void* stream) {

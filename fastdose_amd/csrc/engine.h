@@ -189,13 +189,13 @@ int run_terma(const fd_ctx* ctx, Workspace& w, int options, int n, const float* 
               uint64_t stride, hipStream_t stream);
 int run_cccs(const fd_ctx* ctx, Workspace& w, int options, int n, int max_L, const float* terma, const float* dens,
              float* dose, uint64_t stride, hipStream_t stream);
-// Resampler stages 1-2 (one host synchronisation; a second only when the estimated staging
-// overflows); leaves the offsets in w.  thr > 0 keeps only entries >= thr * (the beamlet's
-// maximum value).
-int run_resample_values(const fd_ctx* ctx, Workspace& w, const fd_beam* beams, int n, const float* dose,
-                        float extent, float thr, uint64_t* nnz_total, hipStream_t stream);
+// The whole resampler into the caller's CSR arrays `out` (offsets always; columns and values
+// only when nnz <= out.capacity), with one host synchronisation (a second only when the estimated
+// staging overflows).  thr > 0 keeps only entries >= thr * (the beamlet's maximum value).
+int run_resample_csr(const fd_ctx* ctx, Workspace& w, const fd_beam* beams, int n, const float* dose, float extent,
+                     float thr, const fd_csr& out, uint64_t* nnz_total, hipStream_t stream);
 int run_resample_compact(const fd_ctx* ctx, Workspace& w, int n, float thr, uint64_t* offsets, uint64_t* columns,
-                         float* values, hipStream_t stream);
+                         float* values, uint64_t capacity, hipStream_t stream);
 
 // The whole path for a batch of beamlets (host descriptors and host fluence, 256 floats each):
 // Terma -> CCCS -> resampling, CSR rows into w.out_cols / w.out_vals, one host synchronisation

@@ -332,29 +332,32 @@ int fetch_totals(const fd_ctx* ctx, Workspace& w, int n, hipStream_t stream, boo
 
 }  // namespace
 
-int run_resample_values(const fd_ctx* ctx, Workspace& w, const fd_beam* beams, int n, const float* dose, float extent,
-                        float thr, uint64_t* nnz_total, hipStream_t stream) {
+int run_resample_csr(const fd_ctx* ctx, Workspace& w, const fd_beam* beams, int n, const float* dose, float extent,
+                     float thr, const fd_csr& out, uint64_t* nnz_total, hipStream_t stream) {
     FD_TRY(reserve_segments(ctx, w, n));
-    // staging sized from the running candidate estimate (no host round trip before the values
-    // pass); the values pass refuses overflowing segments and is re-run once with the exact size
+    // Staging sized from the running candidate estimate, and the copy into the caller's arrays
+    // (skipped on the device when they are too small) issued before the host learns the totals:
+    // one synchronisation, a second pass only when the estimated staging overflows.
     double est = 0;
     for (int i = 0; i < n; i++) est += frustum_voxels(beams[i], ctx->vol);
     if (w.cand_per_beamlet <= 0) w.cand_per_beamlet = 2.0;
     uint64_t cap = std::max<uint64_t>((uint64_t)std::ceil(w.cand_per_beamlet * est), 1024);
-    FD_TRY(w.tmp_cols.reserve(cap));
-    FD_TRY(w.tmp_vals.reserve(cap));
-    FD_TRY(launch_count(ctx, w, n, stream));
-    FD_TRY(launch_values(ctx, w, n, dose, extent, thr, cap, stream));
-    FD_TRY(fetch_totals(ctx, w, n, stream, false));
-    FD_HIP(hipStreamSynchronize(stream));
-    const uint64_t ncand = w.host_totals[0];
-    if (w.host_totals[2]) {
-        cap = ncand;
+    const uint64_t capacity = (out.columns && out.values) ? out.capacity : 0;
+    auto pass = [&]() -> int {
         FD_TRY(w.tmp_cols.reserve(cap));
         FD_TRY(w.tmp_vals.reserve(cap));
         FD_TRY(launch_values(ctx, w, n, dose, extent, thr, cap, stream));
+        FD_TRY(run_resample_compact(ctx, w, n, thr, out.offsets, out.columns, out.values, capacity, stream));
         FD_TRY(fetch_totals(ctx, w, n, stream, false));
         FD_HIP(hipStreamSynchronize(stream));
+        return FD_OK;
+    };
+    FD_TRY(launch_count(ctx, w, n, stream));
+    FD_TRY(pass());
+    const uint64_t ncand = w.host_totals[0];
+    if (w.host_totals[2]) {
+        cap = ncand;
+        FD_TRY(pass());
         if (w.host_totals[2]) {
             set_error("resampler staging overflow with exact sizing");
             return FD_ERR_HIP;
@@ -366,11 +369,11 @@ int run_resample_values(const fd_ctx* ctx, Workspace& w, const fd_beam* beams, i
 }
 
 int run_resample_compact(const fd_ctx* ctx, Workspace& w, int n, float thr, uint64_t* offsets, uint64_t* columns,
-                         float* values, hipStream_t stream) {
+                         float* values, uint64_t capacity, hipStream_t stream) {
     w.mark(FD_T_RESAMPLE_COMPACT, 0, stream);
     hipError_t e = launch_resample_compact(n, (int)ctx->vol.dim[2], w.cand_off.ptr, w.nnz_off.ptr, w.nnz_count.ptr,
                                            w.tmp_cols.ptr, w.tmp_vals.ptr, thr > 0.0f ? w.rowmax.ptr : nullptr, thr,
-                                           offsets, columns, values, stream);
+                                           offsets, columns, values, capacity, stream);
     w.mark(FD_T_RESAMPLE_COMPACT, 1, stream);
     if (e != hipSuccess) {
         set_error(std::string("resample compact: ") + hipGetErrorString(e));
@@ -418,7 +421,7 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
         FD_TRY(run_cccs(ctx, w, options, n, maxL, w.terma.ptr, w.dens_bev.ptr, w.dose_bev.ptr, stride, st));
         FD_TRY(launch_count(ctx, w, n, st));
         FD_TRY(launch_values(ctx, w, n, w.dose_bev.ptr, extent, thr, cap, st));
-        FD_TRY(run_resample_compact(ctx, w, n, thr, w.rowoff.ptr, w.out_cols.ptr, w.out_vals.ptr, st));
+        FD_TRY(run_resample_compact(ctx, w, n, thr, w.rowoff.ptr, w.out_cols.ptr, w.out_vals.ptr, cap, st));
         FD_TRY(fetch_totals(ctx, w, n, st, true));
         return FD_OK;
     };
@@ -469,7 +472,7 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
         FD_TRY(w.out_cols.reserve(cap));
         FD_TRY(w.out_vals.reserve(cap));
         FD_TRY(launch_values(ctx, w, n, w.dose_bev.ptr, extent, thr, cap, stream));
-        FD_TRY(run_resample_compact(ctx, w, n, thr, w.rowoff.ptr, w.out_cols.ptr, w.out_vals.ptr, stream));
+        FD_TRY(run_resample_compact(ctx, w, n, thr, w.rowoff.ptr, w.out_cols.ptr, w.out_vals.ptr, cap, stream));
         FD_TRY(fetch_totals(ctx, w, n, stream, true));
         FD_HIP(hipStreamSynchronize(stream));
         if (w.host_totals[2]) {

@@ -54,11 +54,12 @@ hipError_t launch_resample_threshold_count(int n, int Z, const uint64_t* cand_of
                                            const float* tmp_vals, const uint32_t* rowmax_bits, float thr,
                                            uint32_t* keep_count, hipStream_t stream);
 // Stage 3: copy the compacted nonzeros into the CSR arrays (with rowmax_bits and thr > 0 only
-// those >= thr * rowmax); row offsets from nnz_off.
+// those >= thr * rowmax); row offsets from nnz_off.  When the total (nnz_off[n Z]) exceeds
+// `capacity` entries, the columns and values are left untouched (the offsets are still written).
 hipError_t launch_resample_compact(int n, int Z, const uint64_t* cand_off, const uint64_t* nnz_off,
                                    const uint32_t* nnz_count, const uint64_t* tmp_cols, const float* tmp_vals,
                                    const uint32_t* rowmax_bits, float thr, uint64_t* offsets, uint64_t* columns,
-                                   float* values, hipStream_t stream);
+                                   float* values, uint64_t capacity, hipStream_t stream);
 
 // Exclusive scan of count[0..N) (u32) into off[0..N) (u64) plus total[0] = sum.  With scratch
 // (scan_scratch_size(N) u64, nullable) large inputs take a three-pass multi-workgroup scan.

@@ -350,8 +350,10 @@ __global__ __launch_bounds__(256) void resample_threshold_count_kernel(int Z, co
 __global__ __launch_bounds__(256) void resample_compact_threshold_kernel(
     int Z, const uint64_t* __restrict__ cand_off, const uint64_t* __restrict__ nnz_off,
     const uint32_t* __restrict__ nnz_count, const uint64_t* __restrict__ tmp_cols, const float* __restrict__ tmp_vals,
-    const uint32_t* __restrict__ rowmax_bits, float thr, uint64_t* __restrict__ columns, float* __restrict__ values) {
+    const uint32_t* __restrict__ rowmax_bits, float thr, const uint64_t* __restrict__ total, uint64_t capacity,
+    uint64_t* __restrict__ columns, float* __restrict__ values) {
     __shared__ uint32_t s_wave[256 / 64 + 1];
+    if (total[0] > capacity) return;  // the caller's arrays are too small: write nothing
     const size_t seg = (size_t)blockIdx.y * Z + blockIdx.x;
     const float cut = thr * __uint_as_float(rowmax_bits[blockIdx.y]);
     const uint32_t cnt = nnz_count[seg];
@@ -376,8 +378,10 @@ __global__ __launch_bounds__(256) void resample_compact_kernel(int Z, const uint
                                                                const uint32_t* __restrict__ nnz_count,
                                                                const uint64_t* __restrict__ tmp_cols,
                                                                const float* __restrict__ tmp_vals,
-                                                               uint64_t* __restrict__ columns,
+                                                               const uint64_t* __restrict__ total,
+                                                               uint64_t capacity, uint64_t* __restrict__ columns,
                                                                float* __restrict__ values) {
+    if (total[0] > capacity) return;  // the caller's arrays are too small: write nothing
     const size_t seg = (size_t)blockIdx.y * Z + blockIdx.x;
     const uint32_t cnt = nnz_count[seg];
     const uint64_t src = cand_off[seg], dst = nnz_off[seg];
@@ -451,15 +455,16 @@ hipError_t launch_resample_threshold_count(int n, int Z, const uint64_t* cand_of
 hipError_t launch_resample_compact(int n, int Z, const uint64_t* cand_off, const uint64_t* nnz_off,
                                    const uint32_t* nnz_count, const uint64_t* tmp_cols, const float* tmp_vals,
                                    const uint32_t* rowmax_bits, float thr, uint64_t* offsets, uint64_t* columns,
-                                   float* values, hipStream_t stream) {
+                                   float* values, uint64_t capacity, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     dim3 grid(Z, n), block(256);
+    const uint64_t* total = nnz_off + (size_t)n * Z;
     if (rowmax_bits && thr > 0.0f)
         hipLaunchKernelGGL(resample_compact_threshold_kernel, grid, block, 0, stream, Z, cand_off, nnz_off, nnz_count,
-                           tmp_cols, tmp_vals, rowmax_bits, thr, columns, values);
+                           tmp_cols, tmp_vals, rowmax_bits, thr, total, capacity, columns, values);
     else
         hipLaunchKernelGGL(resample_compact_kernel, grid, block, 0, stream, Z, cand_off, nnz_off, nnz_count, tmp_cols,
-                           tmp_vals, columns, values);
+                           tmp_vals, total, capacity, columns, values);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // nnz_off has n*Z entries followed by the total at index n*Z

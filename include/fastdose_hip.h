@@ -219,7 +219,8 @@ int fd_cccs(fd_ctx* ctx, const fd_beam* beams, int n, const float* terma_dev, co
 /* BEV2PVCSInterp + MatCSREnsemble::addMat (IMRTgeom.cuh:14-33, IMRTDoseMatEns.cuh:30): 5^3
  * point-sampled BEV -> PVCS resampling of n beamlets straight into CSR rows (no dense
  * intermediate).  out->offsets must hold n+1 entries; if out->capacity < nnz the call returns
- * FD_ERR_CAPACITY with out->nnz set (two-phase use). */
+ * FD_ERR_CAPACITY with out->nnz set (two-phase use; the offsets are written either way, the
+ * columns and values only on success). */
 int fd_resample_csr(fd_ctx* ctx, const fd_beam* beams, int n, const float* dose_bev_dev, uint64_t stride,
                     float extent, fd_csr* out, void* stream);
 
