@@ -922,6 +922,28 @@ __global__ __launch_bounds__(256) void cone_sum_kernel(const BeamletDev* __restr
     dose[B.bev_off + i] = acc;
 }
 
+// The same, four values per thread, when the stride and the dose buffer are 16-byte aligned.
+__global__ __launch_bounds__(256) void cone_sum4_kernel(const BeamletDev* __restrict__ beams, int n_fwd,
+                                                       const float* __restrict__ partial,
+                                                       float* __restrict__ dose, uint64_t stride) {
+    const int b = blockIdx.y;
+    const BeamletDev& B = beams[b];
+    const uint64_t count4 = (uint64_t)B.L * (kNpix / 4);
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count4) return;
+    const float4* src = reinterpret_cast<const float4*>(partial + (uint64_t)b * n_fwd * stride) + i;
+    const uint64_t stride4 = stride / 4;
+    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int t = 0; t < n_fwd; t++) {
+        const float4 v = src[(uint64_t)t * stride4];
+        acc.x += v.x;
+        acc.y += v.y;
+        acc.z += v.z;
+        acc.w += v.w;
+    }
+    reinterpret_cast<float4*>(dose + B.bev_off)[i] = acc;
+}
+
 }  // namespace
 
 hipError_t launch_cccs(const BeamletDev* beams, int n, int max_L, const ConeTable* cones_dev, int n_fwd,
@@ -955,8 +977,14 @@ hipError_t launch_cccs(const BeamletDev* beams, int n, int max_L, const ConeTabl
         if (e != hipSuccess) return e;
     }
     if (ev) (void)hipEventRecord(ev[1], stream);
-    dim3 g2((unsigned)(((uint64_t)max_L * kNpix + 255) / 256), (unsigned)n), b2(256);
-    hipLaunchKernelGGL(cone_sum_kernel, g2, b2, 0, stream, beams, n_fwd, partial, dose, stride);
+    if (stride % 4 == 0 && reinterpret_cast<uintptr_t>(dose) % 16 == 0 &&
+        reinterpret_cast<uintptr_t>(partial) % 16 == 0) {
+        dim3 g2((unsigned)(((uint64_t)max_L * (kNpix / 4) + 255) / 256), (unsigned)n), b2(256);
+        hipLaunchKernelGGL(cone_sum4_kernel, g2, b2, 0, stream, beams, n_fwd, partial, dose, stride);
+    } else {
+        dim3 g2((unsigned)(((uint64_t)max_L * kNpix + 255) / 256), (unsigned)n), b2(256);
+        hipLaunchKernelGGL(cone_sum_kernel, g2, b2, 0, stream, beams, n_fwd, partial, dose, stride);
+    }
     if (ev) (void)hipEventRecord(ev[2], stream);
     return hipGetLastError();
 }
