@@ -16,7 +16,9 @@ namespace fdi {
 int Workspace::init() {
     FD_HIP(hipEventCreateWithFlags(&stage_done, hipEventDisableTiming));
     FD_HIP(hipHostMalloc((void**)&host_totals, 4 * sizeof(uint64_t), hipHostMallocDefault));
-    for (auto& e : ev) FD_HIP(hipEventCreate(&e));
+    // kernel timers only time: no system-scope fence (cache write-back and invalidation) at each
+    // record; results are read after the stream has been synchronised
+    for (auto& e : ev) FD_HIP(hipEventCreateWithFlags(&e, hipEventDefault | hipEventDisableSystemFence));
     return FD_OK;
 }
 
