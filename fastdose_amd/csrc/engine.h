@@ -152,7 +152,7 @@ struct fd_ctx {
     float threshold = 0.0f;  // relative CSR threshold (0: drop exact zeros only)
     int n_cu = 256;          // compute units of the device
     bool rcp_exact = false;  // the resampler's reciprocal is the IEEE quotient on this device
-    fd_tuning tune{8192, 0, 8, 0.0f};  // launch parameters (fd_autotune; Terma segments also plain)
+    fd_tuning tune{8192, 0, 16, 0.0f};  // launch parameters (fd_autotune; Terma segments also plain)
     fdi::Workspace ws;
     fdi::DevBuf<fdi::FlagBeamDev> flag_desc;  // beamlet-flag kernel inputs / outputs
     fdi::DevBuf<uint8_t> flag_out;

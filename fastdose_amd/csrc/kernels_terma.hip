@@ -14,9 +14,9 @@
 // share cache lines.
 //
 // The radiological path is a prefix sum along each ray, which one lane per ray would walk alone
-// (256 rays per beamlet: one wavefront per SIMD on small batches).  Each ray is cut into 1, 2, 4
-// or 8 segments of slices (a launch parameter; with 8, a beamlet's rays take two workgroups of
-// 128), one wavefront group per segment: pass 1 writes the
+// (256 rays per beamlet: one wavefront per SIMD on small batches).  Each ray is cut into 1, 2, 4,
+// 8 or 16 segments of slices (a launch parameter; with 8 or 16, a beamlet's rays take two
+// workgroups of 128 or four of 64), one wavefront group per segment: pass 1 writes the
 // mean density of its slices, sums rho |step| over its segment and (FP32 path) stores the running
 // in-segment sums of the four samples of each slice in a scratch buffer; after a barrier each
 // segment adds the sum of the segments before it (in order) and pass 2 reads the stored sums back
@@ -33,7 +33,7 @@ namespace fdi {
 namespace {
 
 constexpr int kTermaThreads = 1024;  // workgroup size limit: ray segments x rays per workgroup
-constexpr int kTermaMaxSeg = 8;     // most ray segments (8: two workgroups of 128 rays per beamlet)
+constexpr int kTermaMaxSeg = 16;    // most ray segments (16: four workgroups of 64 rays per beamlet)
 
 // Corner records: for every base voxel (x, y, z) in [0, X] x [0, Y] x [0, Z] of the density with a
 // one-voxel zero border, its eight trilinear corners (x..x+1, y..y+1, z..z+1; x fastest) as two
@@ -217,7 +217,8 @@ hipError_t launch_terma(const BeamletDev* beams, int n, const float* fluence, co
                         const fd_volume& vol, const SpectrumDev& spec, float rpl64_len, float* terma,
                         float* dens_bev, float* scratch, int segments, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    if (segments != 1 && segments != 2 && segments != 4 && segments != kTermaMaxSeg) return hipErrorInvalidValue;
+    if (segments != 1 && segments != 2 && segments != 4 && segments != 8 && segments != kTermaMaxSeg)
+        return hipErrorInvalidValue;
     const int rays = std::min(kNpix, kTermaThreads / segments);
     dim3 grid(n, kNpix / rays), block(segments * rays);
     const int X = vol.dim[0], Y = vol.dim[1], Z = vol.dim[2];

@@ -255,8 +255,8 @@ int fd_dosemat_add_beams(fd_ctx* ctx, fd_dosemat* m, const fd_beam* beamlets, co
 typedef struct fd_tuning {
     int batch_beamlets;   /* beamlets per batch when fd_dosemat_add_beams / serving are given 0 */
     int graph;            /* 1: replay each batch as one captured HIP graph (FD_OPT_GRAPH) */
-    int terma_segments;   /* Terma ray segments: 1, 2, 4 (wavefront groups of 256 rays) or 8 (two
-                             workgroups of 128 rays per beamlet) */
+    int terma_segments;   /* Terma ray segments: 1, 2, 4 (wavefront groups of 256 rays), 8 or 16
+                             (two workgroups of 128 or four of 64 rays per beamlet) */
     float beamlets_per_s; /* throughput of this setting in the tuning run (0: defaults, not tuned) */
 } fd_tuning;
 
@@ -265,7 +265,7 @@ int fd_ctx_set_tuning(fd_ctx* ctx, const fd_tuning* t);
 
 /* Times trial runs of fd_dosemat_add_beams over the given beams (whole beams in order; a few dozen
  * representative beams suffice): batch sizes of 1024 .. 16384 beamlets, then graph replay on and
- * off, then 2, 4 or 8 Terma segments, each setting run twice and its faster run kept.  The fastest
+ * off, then 4, 8 or 16 Terma segments, each setting run twice and its faster run kept.  The fastest
  * setting becomes the context's tuning and is returned in *out.  Synchronous; the trial matrices
  * are discarded. */
 int fd_autotune(fd_ctx* ctx, const fd_beam* beamlets, const float* fluence, const int* counts, int n_beams,

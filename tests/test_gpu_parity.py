@@ -276,9 +276,9 @@ def test_reciprocal_variants(torch_cuda):
     check_against_oracle(cfg, beams, flu, out[1], range(len(beams)))
 
 
-@pytest.mark.parametrize("segments", [1, 2, 4, 8])
+@pytest.mark.parametrize("segments", [1, 2, 4, 8, 16])
 def test_terma_segments(torch_cuda, segments):
-    """Terma with 1, 2, 4 or 8 ray segments (a tuning parameter) stays within the bar."""
+    """Terma with 1, 2, 4, 8 or 16 ray segments (a tuning parameter) stays within the bar."""
     cfg = synth.small_config(seed=21, dim=(26, 24, 30), fluence_dim=2, angles=(15.0, 50.0, -20.0))
     beams, flu = beamlets_of(cfg)
     with fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel) as ctx:
@@ -310,7 +310,7 @@ def test_autotune(torch_cuda):
         t = ctx.autotune(beamlets, np.concatenate(fls, 0), counts, cfg.extent)
         log(f"autotune {t}")
         assert t["batch_beamlets"] in (1024, 2048, 4096, 8192, 16384) and t["graph"] in (0, 1)
-        assert t["terma_segments"] in (2, 4, 8) and t["beamlets_per_s"] > 0
+        assert t["terma_segments"] in (4, 8, 16) and t["beamlets_per_s"] > 0
         assert ctx.tuning == t
         ref = fd.dose_mat_construction(ctx, cfg.bundles, cfg.flags, cfg.extent)
         dm = fd.dose_mat_construction(ctx, cfg.bundles, cfg.flags, cfg.extent, batched=True, max_chunk=0)
