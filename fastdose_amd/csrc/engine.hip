@@ -102,8 +102,6 @@ int max_long_dim(const fd_beam* beams, int n) {
 
 namespace {
 
-// Pinned staging of the descriptors (and optionally the fluence) of a batch; waits until the
-// previous upload from this workspace has left the staging area.
 // Device descriptors of n beamlets with BEV volumes `stride` floats apart.  Beamlets of one beam
 // share the orientation, so the rotation matrices are computed once per run of equal angles.
 void build_descriptors(const fd_beam* beams, int n, uint64_t stride, BeamletDev* out) {
@@ -118,6 +116,8 @@ void build_descriptors(const fd_beam* beams, int n, uint64_t stride, BeamletDev*
     }
 }
 
+// Pinned staging of the descriptors (beams non-null) and optionally the fluence of a batch; waits
+// until the previous upload from this workspace has left the staging area.
 int stage_inputs(Workspace& w, const fd_beam* beams, int n, uint64_t stride, const float* fluence_host) {
     FD_HIP(hipEventSynchronize(w.stage_done));
     if ((size_t)n > w.stage_cap) {

@@ -190,7 +190,9 @@ int fd_read_kernel(const char* path, int n_phi, fd_kernel* out);
 /* ------------------------------------------------------------------ device context */
 
 /* density_h2d + bind_spectrum + bind_kernel (density.cu:23-58, TermaComp.cu:67-78,
- * DoseComp.cu:25-38): uploads the density volume and tables once. */
+ * DoseComp.cu:25-38): uploads the density volume and tables once.  The stage and batched calls on
+ * one context share its scratch: issue them from one thread, on one stream at a time (a server,
+ * fd_serve_start, gives each of its slots a stream and scratch of its own). */
 int fd_ctx_create(int device, const fd_volume* vol, const float* density_host, const fd_spectrum* spectrum,
                   const fd_kernel* kernel, fd_ctx** out);
 int fd_ctx_destroy(fd_ctx* ctx);
