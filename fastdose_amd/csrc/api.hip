@@ -803,7 +803,8 @@ int fd_ctx_get_tuning(const fd_ctx* ctx, fd_tuning* out) {
 int fd_ctx_set_tuning(fd_ctx* ctx, const fd_tuning* t) {
     FD_TRY(check_ctx(ctx));
     if (!t || t->batch_beamlets < 1 || (t->graph != 0 && t->graph != 1) ||
-        (t->terma_segments != 1 && t->terma_segments != 2 && t->terma_segments != 4 && t->terma_segments != 8 && t->terma_segments != 16)) {
+        (t->terma_segments != 1 && t->terma_segments != 2 && t->terma_segments != 4 && t->terma_segments != 8 &&
+         t->terma_segments != 16)) {
         set_error("tuning: batch_beamlets >= 1, graph 0 or 1, terma_segments 1, 2, 4, 8 or 16");
         return FD_ERR_ARG;
     }
