@@ -129,19 +129,40 @@ __global__ __launch_bounds__(kRowThreads) void resample_count_kernel(const Beaml
                                                                      uint32_t* __restrict__ cand_count) {
     __shared__ uint32_t s_cnt[kCountPlanes];
     const int z0 = blockIdx.x * kCountPlanes, b = blockIdx.y;
-    const int np = min(kCountPlanes, Z - z0);
     const BeamletDev& B = beams[b];
     const double rho = rho_of(vsx, vsy, vsz);
+    // Planes whose voxel centres lie outside the z-range of the widened frustum (the extremes of
+    // its eight vertices, mapped back with the transpose of the rotation, plus 10 um for rounding)
+    // hold no candidate: only the planes in [za, zb) are walked row by row.
+    double zmin = 1e300, zmax = -1e300;
+    {
+        const double cx = 0.5 * kFmap * (double)B.bs[0] / (double)B.sad;
+        const double cz = 0.5 * kFmap * (double)B.bs[1] / (double)B.sad;
+        const double rx = rho * sqrt(1.0 + cx * cx), rz = rho * sqrt(1.0 + cz * cz);
+        const double ylo = (double)B.lim_min - rho;
+        const double yhi = (double)B.lim_min + (double)B.L * (double)B.ls + rho;
+        for (int v = 0; v < 8; v++) {
+            const double by = (v & 1) ? yhi : ylo;
+            const double bx = ((v & 2) ? 1.0 : -1.0) * fabs(cx * by + rx);
+            const double bz = ((v & 4) ? 1.0 : -1.0) * fabs(cz * by + rz);
+            const double qz = (double)B.src[2] + (double)B.Rf[2] * bx + (double)B.Rf[5] * by + (double)B.Rf[8] * bz;
+            zmin = fmin(zmin, qz);
+            zmax = fmax(zmax, qz);
+        }
+    }
+    const int za = max(z0, (int)floor((zmin - 1e-3) / (double)vsz - 0.5));
+    const int zb = min(min(z0 + kCountPlanes, Z), (int)ceil((zmax + 1e-3) / (double)vsz - 0.5) + 1);
     if (threadIdx.x < kCountPlanes) s_cnt[threadIdx.x] = 0u;
     __syncthreads();
+    const int np = max(zb - za, 0);
     for (int i = threadIdx.x; i < np * Y; i += blockDim.x) {
         const int p = i / Y, y = i - p * Y;
         int x0, x1;
-        row_interval(B, y, z0 + p, X, vsx, vsy, vsz, rho, x0, x1);
-        if (x1 > x0) atomicAdd(&s_cnt[p], (uint32_t)(x1 - x0));
+        row_interval(B, y, za + p, X, vsx, vsy, vsz, rho, x0, x1);
+        if (x1 > x0) atomicAdd(&s_cnt[za - z0 + p], (uint32_t)(x1 - x0));
     }
     __syncthreads();
-    if (threadIdx.x < np) cand_count[(size_t)b * Z + z0 + threadIdx.x] = s_cnt[threadIdx.x];
+    if (threadIdx.x < min(kCountPlanes, Z - z0)) cand_count[(size_t)b * Z + z0 + threadIdx.x] = s_cnt[threadIdx.x];
 }
 
 // floor(v) as an integer (v_cvt_flr_i32_f32: out-of-range values saturate)
@@ -255,6 +276,11 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(8, 
     uint32_t* s_x0 = s_rows;
     uint32_t* s_off = s_rows + Y;
     const int z = blockIdx.x, b = blockIdx.y;
+    const size_t seg = (size_t)b * Z + z;
+    if (cand_off[seg + 1] == cand_off[seg]) {  // no candidate in this plane (most planes of a beamlet)
+        if (threadIdx.x == 0) nnz_count[seg] = 0u;
+        return;
+    }
     const BeamletDev& B = beams[b];
     const double rho = rho_of(vsx, vsy, vsz);
 
