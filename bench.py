@@ -462,6 +462,10 @@ def main() -> int:
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
             "traffic": profiled_traffic("cccs_kernel_slab128_traffic.md"),
             "algorithmic": f"{FLOPS_PER_SEGMENT} flop x {nseg} (ray, segment) steps per launch",
+            # the published FP32 peak counts packed (two-wide) FMAs, which this build does not
+            # emit; the non-packed FMA rate is half of it
+            "peak_non_packed": FP32_PEAK_TFLOPS / 2,
+            "frac_of_non_packed": round(achieved / (FP32_PEAK_TFLOPS / 2), 4),
         },
         "cpu_baseline": None,
     }
