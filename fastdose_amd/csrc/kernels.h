@@ -20,7 +20,8 @@ hipError_t launch_expand_density(const float* density, const fd_volume& vol, flo
 // Terma + mean density along the 256 BEV rays of each beamlet (TermaComp.cu:179-253).
 // density_pad: the corner records of launch_expand_density.
 // scratch: 4 floats per BEV value (the layout of terma, 16-byte aligned), overwritten.
-// segments: ray segments per workgroup (1, 2 or 4; 256 threads each).
+// segments: ray segments per ray (1, 2, 4, 8 or 16); a workgroup holds min(256, 1024 / segments)
+// rays times `segments` threads, so with 8 or 16 a beamlet takes two or four workgroups.
 // Beamlets with long_dim * long_spacing > rpl64_len [cm] accumulate the radiological path in FP64.
 hipError_t launch_terma(const BeamletDev* beams, int n, const float* fluence, const float* density_pad,
                         const fd_volume& vol, const SpectrumDev& spec, float rpl64_len, float* terma,
