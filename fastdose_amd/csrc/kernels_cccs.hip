@@ -107,6 +107,40 @@ __device__ __forceinline__ void shift_y(float (&XA)[N], float (&XB)[N], int d, i
     }
 }
 
+// The same for the X state and a third voxel-indexed array that follows the rays (their offsets).
+template <int N>
+__device__ __forceinline__ void shift_y3(float (&XA)[N], float (&XB)[N], float (&Z)[N], int d, int up_lane,
+                                         int down_lane) {
+    for (; d > 0; d--) {  // new[y] = old[y - 1]
+        const float ta = wave_from(XA[N - 1], up_lane);
+        const float tb = wave_from(XB[N - 1], up_lane);
+        const float tz = wave_from(Z[N - 1], up_lane);
+#pragma unroll
+        for (int r = N - 1; r > 0; r--) {
+            XA[r] = XA[r - 1];
+            XB[r] = XB[r - 1];
+            Z[r] = Z[r - 1];
+        }
+        XA[0] = ta;
+        XB[0] = tb;
+        Z[0] = tz;
+    }
+    for (; d < 0; d++) {  // new[y] = old[y + 1]
+        const float ta = wave_from(XA[0], down_lane);
+        const float tb = wave_from(XB[0], down_lane);
+        const float tz = wave_from(Z[0], down_lane);
+#pragma unroll
+        for (int r = 0; r < N - 1; r++) {
+            XA[r] = XA[r + 1];
+            XB[r] = XB[r + 1];
+            Z[r] = Z[r + 1];
+        }
+        XA[N - 1] = ta;
+        XB[N - 1] = tb;
+        Z[N - 1] = tz;
+    }
+}
+
 // Lattice coordinate (within the 16-wide period) of the one ray that crosses the lateral edge in
 // a step from integer cell fb to fn (|fn - fb| <= 1): the ray with offset i crosses iff
 // floor((fb + i) / 16) != floor((fn + i) / 16), i.e. i = -max(fb, fn) mod 16.  -1: none.
@@ -717,9 +751,9 @@ __device__ __forceinline__ void cccs_compute(const CcBlock& K, const CcLds& S, i
         XB[r] = 0.0f;
     }
     // ray offsets of my voxels relative to the midpoint voxel (mx, my), starting at (0, 0):
-    // fix = (x - mx) & 15; iy of slot r = (4 g + r - my) & 15, packed one per byte
+    // fix = (x - mx) & 15, fiy[r] = (4 g + r - my) & 15.  An offset belongs to the ray, so it moves
+    // with the X state when the midpoint voxel moves (a shift by one voxel subtracts 1 mod 16)
     float fix = (float)vx;
-    uint32_t py = (uint32_t)(4 * g) * 0x01010101u + 0x03020100u;
     float fiy[kSlots];
 #pragma unroll
     for (int r = 0; r < kSlots; r++) fiy[r] = (float)(4 * g + r);
@@ -772,11 +806,7 @@ __device__ __forceinline__ void cccs_compute(const CcBlock& K, const CcLds& S, i
                 fix = (float)((vx - (int)((w >> 10) & 15)) & 15);
             }
             if (ddy != 0) {
-                shift_y(XA, XB, ddy, up_lane, down_lane);
-                // (iy - ddy) & 15 per byte: iy + 16 - ddy stays within [9, 39], no carry between bytes
-                py = (py + (uint32_t)(16 - ddy) * 0x01010101u) & 0x0F0F0F0Fu;
-#pragma unroll
-                for (int r = 0; r < kSlots; r++) fiy[r] = (float)((py >> (8 * r)) & 255u);
+                shift_y3(XA, XB, fiy, ddy, up_lane, down_lane);
             }
             const float ddxl = fmaf(ra.x, fix, ra.z);
             const float dxz2 = fmaf(ddxl, ddxl, rb.x);
