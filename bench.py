@@ -342,6 +342,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched 4pi, serving and SpMM legs")
+    ap.add_argument("--only-batched-4pi", action="store_true",
+                    help="run only the batched 4pi leg and print its JSON (for kernel traces of that path)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 256 MiB device copy first (reference for HBM traffic counters)")
     args = ap.parse_args()
@@ -359,6 +361,10 @@ def main() -> int:
     if not torch.cuda.is_available():
         log("no HIP device")
         return 1
+    if args.only_batched_4pi:
+        leg, _ = leg_batched_4pi(fd, synth, torch, np)
+        print(json.dumps({"batched_4pi": leg}), flush=True)
+        return 0
     dev = torch.device("cuda:0")
     cfg = synth.config2()
     beams, flu = fd.beamlets_init(cfg.bundles[0], cfg.flags[0], cfg.volume)
