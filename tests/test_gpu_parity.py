@@ -320,12 +320,15 @@ def test_autotune(torch_cuda):
 
 
 def test_device_beamlet_flags_match_host(torch_cuda):
-    """fd_beamlet_flags (device, many bundles) == fd_beamlet_flags_host, bit for bit."""
+    """fd_beamlet_flags (device, many bundles) == the oracle's flags (orc_beamlet_flags, pinned by its
+    closed-form and brute-force tests) == fd_beamlet_flags_host, bit for bit."""
     torch = torch_cuda
     cfg = synth.fourpi_config("flags-4pi", (60, 56, 48), 1011, 24, (7.0, 5.0, 4.0), fluence_dim=12,
                               with_flags=False)
     segs = cfg.notes["segments"]
     seg_of = cfg.notes["segment_of_beam"]
+    ovol = orc.cast(orc.Volume, cfg.volume)
+    on = 0
     with fd.Context(cfg.volume, cfg.density, cfg.spectrum, cfg.kernel) as ctx:
         for s_idx, mask in enumerate(segs):
             ks = [k for k in range(len(cfg.bundles)) if seg_of[k] == s_idx]
@@ -333,8 +336,10 @@ def test_device_beamlet_flags_match_host(torch_cuda):
             mask_d = torch.from_numpy(np.ascontiguousarray(mask, dtype=np.uint8)).cuda()
             dev = ctx.beamlet_flags(mask_d, bundles)
             for b, f in zip(bundles, dev):
-                host = fd.beamlet_flag_init(b, mask, cfg.volume)
-                assert np.array_equal(host, f)
+                assert np.array_equal(orc.beamlet_flags(orc.cast(orc.Bundle, b), mask, ovol), f)
+                assert np.array_equal(fd.beamlet_flag_init(b, mask, cfg.volume), f)
+                on += int(f.sum())
+        assert on > 0
         # a bundle far outside the volume: all flags 0
         far = synth.make_bundle(np.array([500.0, 500.0, 500.0], np.float32), (0.0, 0.0, 0.0), fluence_dim=4)
         mask_d = torch.from_numpy(np.ascontiguousarray(segs[0], dtype=np.uint8)).cuda()

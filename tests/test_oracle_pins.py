@@ -424,3 +424,437 @@ def test_dense_to_csr_known_matrices():
     assert off.tolist() == [0, 2, 4, 5, 6]
     assert col.tolist() == [2, 3, 0, 3, 2, 0]
     assert val.tolist() == [1.0, 2.0, 1.0, 1.0, 1.0, 2.0]
+
+
+# ---------------------------------------------------------------- trilinear density (TermaComp.cu:222-233)
+
+def _bev_to_pvcs64(th, ph, co):
+    """inverseRotateBeamAtOriginRHS (geom.cu:47-56) as an FP64 matrix: the transpose of the forward
+    rotation, whose columns are the images of the unit vectors under _rot_fp64."""
+    F = np.stack([_rot_fp64(e, th, ph, co) for e in np.eye(3)], 1)
+    return F.T
+
+
+def _ramp_1d(c, n, alpha, beta):
+    """1-D linear texture (texel i centred at i + 0.5, border 0) of the field g(i) = alpha + beta (i + 0.5),
+    i in [0, n), sampled at c: the line itself between the first and last texel centres, falling linearly
+    to 0 over the half texel beyond them."""
+    c = np.asarray(c, dtype=np.float64)
+    g0, gn = alpha + beta * 0.5, alpha + beta * (n - 0.5)
+    out = np.where((c > 0.5) & (c < n - 0.5), alpha + beta * c, 0.0)
+    out = np.where((c > -0.5) & (c <= 0.5), (c + 0.5) * g0, out)
+    out = np.where((c >= n - 0.5) & (c < n + 0.5), (n + 0.5 - c) * gn, out)
+    return out
+
+
+def test_terma_density_ramp_closed_form():
+    """A separable density fx(x) fy(y) fz(z) of three linear ramps: trilinear filtering reproduces it
+    exactly between texel centres and falls to 0 over the half texel at each border (CUDA border
+    addressing), so DensityBEV is the mean of four closed-form samples and the radiological path their
+    running sum.  Anisotropic voxels and distinct ramps catch an axis mix-up; the beam enters and leaves
+    through the border half-texels and one side of its field runs along the x = 0 face."""
+    vol = orc.Volume()
+    dims = (24, 20, 22)
+    vs = tuple(float(np.float32(v)) for v in (0.25, 0.3, 0.2))   # as the FP32 volume holds them
+    for i in range(3):
+        vol.voxel_size[i] = vs[i]
+        vol.dim[i] = dims[i]
+        vol.bbox_start[i] = 0
+        vol.bbox_dim[i] = dims[i]
+    ramps = ((0.6, 0.05), (1.3, -0.04), (0.4, 0.07))       # (alpha, beta) per axis x, y, z
+    g = [ramps[d][0] + ramps[d][1] * (np.arange(dims[d]) + 0.5) for d in range(3)]
+    dens = (g[2][:, None, None] * g[1][None, :, None] * g[0][None, None, :]).astype(np.float32)
+    # the field is stored in FP32: use the stored values' own ramps (exact in FP64 up to their rounding)
+    assert np.allclose(dens, g[2][:, None, None] * g[1][None, :, None] * g[0][None, None, :], rtol=1e-6)
+    bm = orc.Beam()
+    iso = (0.9, 3.0, 2.2)
+    for i in range(3):
+        bm.isocenter[i] = iso[i]
+    bm.beamlet_size[0] = bm.beamlet_size[1] = 0.125
+    bm.fmap_size[0] = bm.fmap_size[1] = 16
+    bm.sad = 100.0
+    bm.long_spacing = 0.25
+    bm.angles[0], bm.angles[1], bm.angles[2] = 0.35, 0.6, 0.2
+    assert orc.calc_range(bm, vol) == 0
+    spec = orc.cast(orc.Spectrum, synth.synthetic_spectrum(1009))
+    fl = np.random.default_rng(4).uniform(0.5, 1.5, 256).astype(np.float32)
+    T, R = orc.terma(bm, fl, vol, dens, spec)
+    L = bm.long_dim
+    Mi = _bev_to_pvcs64(float(bm.angles[0]), float(bm.angles[1]), float(bm.angles[2]))
+    sad, ls, lm = float(bm.sad), float(bm.long_spacing), float(bm.lim_min)
+    src = vec(bm.source)
+    dt = ls / (sad * 4)
+    t = lm / sad + np.arange(1, 4 * L + 1) * dt
+    n = spec.n
+    mu = np.array(spec.mu[:n], np.float64)
+    w = np.array(spec.fluence[:n], np.float64) * np.array(spec.energy[:n], np.float64) * mu
+    border = 0
+    for p in range(256):
+        ix, iy = p % 16, p // 16
+        pv = Mi @ np.array([(ix + 0.5 - 8) * 0.125, sad, (iy + 0.5 - 8) * 0.125])
+        c = (src[None, :] + pv[None, :] * t[:, None]) / np.array(vs)[None, :]
+        f = [_ramp_1d(c[:, d], dims[d], *ramps[d]) for d in range(3)]
+        rho = f[0] * f[1] * f[2]
+        inner = np.all((c > 0.5) & (c < np.array(dims) - 0.5), axis=1)
+        border += int(((rho > 0) & ~inner).sum())
+        assert np.allclose(R[:, p], rho.reshape(L, 4).mean(1), rtol=2e-6, atol=1e-9)
+        rpl = np.cumsum(rho) * np.linalg.norm(pv) * dt
+        terms = (w[None, :] * np.exp(-mu[None, :] * rpl[:, None])).sum(1) / t ** 2
+        assert np.allclose(T[:, p], fl[p] * terms.reshape(L, 4).mean(1), rtol=2e-6, atol=0)
+    assert border > 500  # the border half-texels are exercised
+
+
+# ---------------------------------------------------------------- CCCS per-ray transcription (DoseComp.cu:217-392)
+
+def _cccs_per_ray(bm, T, R, kern):
+    """DoseComp.cu:265-390 restated ray by ray in FP64, as each reference thread runs it: its own
+    coordinates (shared lattice point + (idx_x, idx_y)), the d_calcCoords fmod wrap of its midpoint,
+    d_calcLineSeg of its own endpoints, and its own floor(c / 16) edge test.  The shared lattice point
+    (d_nextPoint) and cone direction come from the oracle's primitives, which the next-point and
+    angle tests above pin; g == 1 (DoseComp.cu:167-175).  Returns the dose and the number of steps
+    whose midpoint lies within 1e-6 of a voxel boundary (where an FP32 midpoint may round across it)."""
+    fx, fy = int(bm.fmap_size[0]), int(bm.fmap_size[1])
+    npix = fx * fy
+    L = int(bm.long_dim)
+    ix = np.arange(npix) % fx
+    iy = np.arange(npix) // fx
+    bs = (float(bm.beamlet_size[0]), float(bm.beamlet_size[1]))
+    ls, lm, sad = float(bm.long_spacing), float(bm.lim_min), float(bm.sad)
+    eps_f32 = float(np.float32(1e-4))
+
+    def phys(x, y, z):
+        f = (z * ls + lm) / sad
+        return np.stack([(x - fx * 0.5) * bs[0] * f, (y - fy * 0.5) * bs[1] * f, np.broadcast_to(z * ls, x.shape)], 1)
+
+    dose = np.zeros((L, npix))
+    near = 0
+    d = (C.c_float * 3)()
+    inv = (C.c_float * 3)()
+    out = (C.c_float * 3)()
+    for t in range(kern.n_theta):
+        th = kern.theta_mid[t]
+        if th > math.pi / 2:
+            continue
+        A, a, B, b = (float(kern.A[t]), float(kern.a[t]), float(kern.B[t]), float(kern.b[t]))
+        nphi = kern.n_phi
+        XA = np.zeros((nphi, npix))
+        XB = np.zeros((nphi, npix))
+        dirs, base = [], []
+        for f in range(nphi):
+            orc.lib().orc_cone_direction(th, kern.phi[f], C.byref(bm), d, inv)
+            dirs.append((f3(*d), f3(*inv)))
+            base.append(np.array([0.5, 0.5, 0.0], dtype=np.float32))
+        while True:
+            s = int(math.floor(float(base[0][2]) + 0.5))
+            if s >= L:
+                break
+            Ts, Rs = T[s], R[s]
+            SD = np.zeros(npix)
+            cumu = None
+            for f in range(nphi):
+                cumu = np.zeros(npix)
+                while True:
+                    flag = orc.lib().orc_next_point(f3(*base[f]), dirs[f][0], dirs[f][1], out)
+                    nb = np.array([out[0], out[1], out[2]], dtype=np.float32)
+                    b0 = [float(base[f][0]) + ix, float(base[f][1]) + iy]
+                    n0 = [float(nb[0]) + ix, float(nb[1]) + iy]
+                    mx, my = (b0[0] + n0[0]) * 0.5, (b0[1] + n0[1]) * 0.5
+                    for m_ in (mx, my):
+                        fr = m_ - np.floor(m_)
+                        near += int(np.count_nonzero((fr < 1e-6) | (fr > 1 - 1e-6)))
+                    wx = np.fmod(mx, fx)
+                    wx = np.where(wx < 0, wx + fx, wx)
+                    wy = np.fmod(my, fy)
+                    wy = np.where(wy < 0, wy + fy, wy)
+                    m = np.floor(wx).astype(int) + fx * np.floor(wy).astype(int)
+                    seg = np.linalg.norm(phys(n0[0], n0[1], float(nb[2])) - phys(b0[0], b0[1], float(base[f][2])),
+                                         axis=1)
+                    p_i = seg * Rs[m]
+                    np.add.at(SD, m, (A / a * XA[f] + B / b * XB[f]) * p_i)
+                    np.add.at(cumu, m, p_i)
+                    ea, eb = np.exp(-a * p_i), np.exp(-b * p_i)
+                    xa = Ts[m] * (1 - ea) + ea * XA[f]
+                    xb = Ts[m] * (1 - eb) + eb * XB[f]
+                    crossed = (np.floor(b0[0] / fx) != np.floor(n0[0] / fx)) | \
+                              (np.floor(b0[1] / fy) != np.floor(n0[1] / fy))
+                    xa[crossed] = 0.0
+                    xb[crossed] = 0.0
+                    XA[f], XB[f] = xa, xb
+                    base[f] = nb
+                    if flag:
+                        break
+            dose[s] += SD / (cumu + eps_f32)
+    return dose, near
+
+
+@pytest.mark.parametrize("L,n_phi,seed", [(6, 8, 1), (9, 5, 2), (7, 12, 3)])
+def test_cccs_per_ray_transcription(L, n_phi, seed):
+    """Every forward cone of the configs' kernel, including the near-horizontal one (tan theta = 5:
+    ~14 segments per slice, many rays crossing the BEV edge each slice), on a beamlet with unequal
+    lateral spacings: the oracle's shared-lattice CCCS (one wrapped shift per step, edge resets from
+    integer floors) equals the per-ray transcription of the reference kernel."""
+    rng = np.random.default_rng(seed)
+    bm = orc.Beam()
+    bm.beamlet_size[0], bm.beamlet_size[1] = 0.125, 0.11
+    bm.fmap_size[0] = bm.fmap_size[1] = 16
+    bm.sad, bm.lim_min, bm.long_spacing, bm.long_dim = 100.0, 93.0, 0.25, L
+    kern = orc.cast(orc.Kernel, synth.synthetic_kernel(1000 + seed, n_phi=n_phi))
+    T = rng.uniform(0.0, 1.0, (L, 256))
+    R = rng.uniform(0.2, 1.8, (L, 256))
+    D, nseg = orc.cccs(bm, T, R, kern)
+    want, near = _cccs_per_ray(bm, T, R, kern)
+    assert near == 0  # no midpoint within FP32 reach of a voxel boundary: the index maps must agree
+    assert nseg > 40 * L * 256 * n_phi / 8
+    assert np.allclose(D, want, rtol=1e-11, atol=1e-14)
+    # the edge resets matter here: without them the dose differs
+    assert D.max() > 0
+
+
+# ---------------------------------------------------------------- PTV flags (IMRTBeamBundle.cu:269-316)
+
+def _flags_brute(bundle, mask, vol, delta=0.0):
+    """d_beamletFlagInit in FP64: 3 x 3 lateral samples per depth step i < long_dim of a beam spanning
+    the bundle's fluence map, at source + R^-1 (x, sad, z) SXD / sad; a beamlet is on when the linear
+    texture of the mask is positive at a sample, i.e. some texel within one texel pitch (per axis) of
+    the sample is set.  delta widens (> 0) or narrows (< 0) that pitch, to find beamlets whose flag
+    hangs on rounding."""
+    bm = orc.Beam()
+    for i in range(3):
+        bm.isocenter[i] = bundle.isocenter[i]
+        bm.angles[i] = bundle.angles[i]
+    bm.beamlet_size[0], bm.beamlet_size[1] = bundle.beamlet_size[0], bundle.beamlet_size[1]
+    bm.fmap_size[0], bm.fmap_size[1] = bundle.fluence_dim[0], bundle.fluence_dim[1]
+    bm.sad = bundle.sad
+    bm.long_spacing = bundle.long_spacing
+    assert orc.calc_range(bm, vol) == 0
+    fx, fy = int(bundle.fluence_dim[0]), int(bundle.fluence_dim[1])
+    Mi = _bev_to_pvcs64(float(bm.angles[0]), float(bm.angles[1]), float(bm.angles[2]))
+    src = vec(bm.source)
+    vs = np.array([float(vol.voxel_size[i]) for i in range(3)])
+    dims = np.array([int(vol.dim[i]) for i in range(3)])
+    sad = float(bm.sad)
+    sxd = float(bm.lim_min) + np.arange(int(bm.long_dim)) * float(bm.long_spacing)
+    off = (np.arange(3) + 0.5) / 3
+    bx = np.arange(fx * fy) % fx
+    by = np.arange(fx * fy) // fx
+    X = (bx[:, None] + off[None, :] - fx * 0.5) * float(bm.beamlet_size[0])       # (nb, 3)
+    Z = (by[:, None] + off[None, :] - fy * 0.5) * float(bm.beamlet_size[1])
+    disp = np.zeros((fx * fy, 3, 3, 3))
+    disp[:, :, :, 0] = X[:, :, None]
+    disp[:, :, :, 1] = sad
+    disp[:, :, :, 2] = Z[:, None, :]
+    pv = disp @ Mi.T                                                                # (nb, j, k, 3)
+    c = (src + pv[:, :, :, None, :] * (sxd / sad)[None, None, None, :, None]) / vs  # (nb, j, k, i, 3)
+    c = c.reshape(fx * fy, -1, 3)
+    pad = np.zeros(tuple(dims[::-1] + 6), dtype=bool)
+    pad[3:-3, 3:-3, 3:-3] = mask.reshape(dims[::-1]) > 0
+    t0 = np.floor(c - 0.5).astype(int)
+    on = np.zeros(c.shape[:2], dtype=bool)
+    for ox in (-1, 0, 1, 2):
+        for oy in (-1, 0, 1, 2):
+            for oz in (-1, 0, 1, 2):
+                t = t0 + np.array([ox, oy, oz])
+                ok = np.all(np.abs(c - (t + 0.5)) < 1.0 + delta, axis=2)
+                t = np.clip(t, -3, dims + 2) + 3
+                on |= ok & pad[t[:, :, 2], t[:, :, 1], t[:, :, 0]]
+    return on.any(1).astype(np.uint8), bm
+
+
+def _flag_bundle(iso, angles, fd=10, bs=0.5):
+    b = orc.Bundle()
+    for i in range(3):
+        b.isocenter[i] = iso[i]
+        b.angles[i] = angles[i]
+    b.beamlet_size[0] = b.beamlet_size[1] = bs
+    b.fluence_dim[0] = b.fluence_dim[1] = fd
+    b.sad = 100.0
+    b.long_spacing = 0.25
+    b.sub_dim[0] = b.sub_dim[1] = 16
+    b.sub_on[0] = b.sub_on[1] = 4
+    return b
+
+
+def test_beamlet_flags_box_closed_form():
+    """Gantry 0: sample (bx, j, by, k) at depth SXD sits at iso + (x_j, SXD - SAD, z_k) with
+    x_j = (bx + (j + .5)/3 - 5) bs SXD/SAD.  A box PTV of voxels [x0, x1) x [y0, y1) x [z0, z1) has a
+    positive linear texture on the open box (x0 - .5, x1 + .5) ... in voxel units, so a beamlet is on
+    exactly when, over the depths whose y lies in the box, one of its x samples and one of its z
+    samples sweep into the box -- a rectangle of beamlets."""
+    vol = orc.cast(orc.Volume, synth.make_volume((40, 40, 40)))
+    mask = np.zeros((40, 40, 40), dtype=np.uint8)
+    box = ((14, 26), (15, 25), (16, 24))   # x, y, z voxel ranges
+    mask[box[2][0]:box[2][1], box[1][0]:box[1][1], box[0][0]:box[0][1]] = 1
+    bundle = _flag_bundle((5.0, 5.0, 5.0), (0.0, 0.0, 0.0))
+    brute, bm = _flags_brute(bundle, mask, vol)
+    vs = 0.25
+    sad = 100.0
+    src_y = float(bm.source[1])
+    sxd = float(bm.lim_min) + np.arange(int(bm.long_dim)) * 0.25
+    yv = (src_y + sxd) / vs
+    ins = (yv > box[1][0] - 0.5) & (yv < box[1][1] + 0.5)
+    f_lo, f_hi = sxd[ins].min() / sad, sxd[ins].max() / sad
+    want = np.zeros(100, dtype=np.uint8)
+    for b in range(100):
+        bx, by = b % 10, b // 10
+        okx = okz = False
+        for j in range(3):
+            o = (bx + (j + 0.5) / 3 - 5) * 0.5
+            lo, hi = sorted((o * f_lo, o * f_hi))
+            okx |= lo + 5.0 < (box[0][1] + 0.5) * vs and hi + 5.0 > (box[0][0] - 0.5) * vs
+            o = (by + (j + 0.5) / 3 - 5) * 0.5
+            lo, hi = sorted((o * f_lo, o * f_hi))
+            okz |= lo + 5.0 < (box[2][1] + 0.5) * vs and hi + 5.0 > (box[2][0] - 0.5) * vs
+        want[b] = okx and okz
+    assert want.reshape(10, 10)[2:8, 1:9].all() and want.sum() == 48   # bx in [1, 8], by in [2, 7]
+    assert (brute == want).all()
+    assert (_flags_brute(bundle, mask, vol, delta=0.02)[0] == want).all()   # no flag near a rounding edge
+    assert (_flags_brute(bundle, mask, vol, delta=-0.02)[0] == want).all()
+    assert (orc.beamlet_flags(bundle, mask, vol) == want).all()
+
+
+def test_beamlet_flags_depth_range_end():
+    """Depth samples are SXD = lim_min + i * long_spacing for i < long_dim (IMRTBeamBundle.cu:289-291):
+    with the bounding box shorter than the volume, a PTV slab beyond the box is flagged exactly when
+    its texture support starts before the last sample.  The slab is placed one texel either side of
+    that sample (with margins far above FP32 rounding), so a depth grid shifted by one step flips it."""
+    vol = orc.cast(orc.Volume, synth.make_volume((40, 48, 40), bbox_dim=(40, 32, 40)))
+    bundle = _flag_bundle((5.0, 4.0, 5.0), (0.0, 0.0, 0.0))
+    _, bm = _flags_brute(bundle, np.zeros((40, 48, 40), dtype=np.uint8), vol)
+    L = int(bm.long_dim)
+    c_last = (float(bm.source[1]) + float(bm.lim_min) + (L - 1) * 0.25) / 0.25
+    s_far = math.floor(c_last + 0.5) + 1            # support (s_far - 0.5, ...) starts past the last sample
+    assert 0.01 < s_far - 0.5 - c_last < 0.99
+    for start, want in ((s_far, 0), (s_far - 1, 1)):
+        mask = np.zeros((40, 48, 40), dtype=np.uint8)
+        mask[:, start:start + 4, :] = 1
+        brute, _ = _flags_brute(bundle, mask, vol)
+        assert (brute == want).all()
+        assert (orc.beamlet_flags(bundle, mask, vol) == want).all()
+
+
+@pytest.mark.parametrize("angles", [(0.7, -0.4, 0.3), (2.2, 1.1, -0.6), (1.5708, 0.0, 0.0)])
+def test_beamlet_flags_brute_force(angles):
+    """Rotated bundles through an ellipsoidal PTV: the oracle's FP32 flags equal the FP64 brute force on
+    every beamlet whose flag does not hang on a sample within 1e-3 texel of a texel-weight edge."""
+    vol = orc.cast(orc.Volume, synth.make_volume((36, 40, 32)))
+    z, y, x = np.meshgrid(*(np.arange(n) + 0.5 for n in (32, 40, 36)), indexing="ij")
+    mask = (((x - 17) / 7.0) ** 2 + ((y - 21) / 9.0) ** 2 + ((z - 15) / 6.0) ** 2 <= 1.0).astype(np.uint8)
+    bundle = _flag_bundle((4.4, 5.1, 3.9), angles, fd=12, bs=0.5)
+    exact, _ = _flags_brute(bundle, mask, vol)
+    wide, _ = _flags_brute(bundle, mask, vol, delta=1e-3)
+    narrow, _ = _flags_brute(bundle, mask, vol, delta=-1e-3)
+    sure = wide == narrow
+    got = orc.beamlet_flags(bundle, mask, vol)
+    assert sure.sum() >= 140 and 10 < exact.sum() < 130
+    assert (got[sure] == exact[sure]).all()
+
+
+# ---------------------------------------------------------------- beamletInit invariants (IMRTBeamBundle.cu:341-569)
+
+def _n(v):
+    return v / np.linalg.norm(v)
+
+
+def _angle(u, v):
+    return math.acos(max(-1.0, min(1.0, float(np.dot(_n(u), _n(v))))))
+
+
+def _beamlet_geometry(bundle, ix, iy):
+    """The beamlet's pyramid in the bundle's BEV, FP64: the corner rays (reference order), the two
+    diagonal bisectors, and for each the in-plane distance of the nearest corner (IMRTBeamBundle.cu:
+    359-421 define the canonical pyramid by these)."""
+    bs = float(bundle.beamlet_size[0]), float(bundle.beamlet_size[1])
+    hx = bundle.fluence_dim[0] * bs[0] * 0.5
+    hy = bundle.fluence_dim[1] * bs[1] * 0.5
+    sad = float(bundle.sad)
+    v = [np.array([(ix + dx) * bs[0] - hx, sad, (iy + dy) * bs[1] - hy]) for dx, dy in ((0, 0), (1, 0), (1, 1), (0, 1))]
+    v = [_n(a) for a in v]
+    out = {}
+    for case, (i, j), others in ((1, (0, 2), (0, 1, 3)), (2, (1, 3), (0, 1, 2))):
+        c = _n(v[i] + v[j])
+        proj = [a / np.dot(a, c) - c for a in v]
+        out[case] = (c, proj, min(np.linalg.norm(proj[k]) for k in others))
+    return v, out
+
+
+def test_beamlet_init_canonical_pyramid():
+    """For random bundles and all their beamlets, the canonical beamlet reproduces the geometry of the
+    beamlet's pyramid (checked in FP64 from the definitions, not from the oracle's arithmetic):
+      * its axis R'^-1 (0, 1, 0) is the diagonal bisector of the beamlet's corner rays (the case with the
+        larger nearest-corner distance), so it aims at the beamlet's centre to within the bisector's
+        offset;
+      * its source iso' + R'^-1 (0, -sad', 0) is the bundle's source (sad' = SAD / axis.y);
+      * its BEV -z axis is the in-plane direction toward corner 0 (case 1: the reference's v0 + v0, Q4)
+        or toward the midpoint of corners 0 and 1 (case 2);
+      * its field (sub_on x beamlet_size') has the sides of the rectangle whose half-diagonals reach the
+        nearest corner distance toward corners 0 and 1; in case 2 the field's corner rays are that
+        rectangle's corners;
+      * the sub-fluence is the centred sub_on x sub_on block."""
+    rng = np.random.default_rng(17)
+    vol = orc.cast(orc.Volume, synth.make_volume((80, 80, 80)))
+    seen = {1: 0, 2: 0, "mirror": 0, "tolerance": 0}
+    for trial in range(6):
+        ang = rng.uniform([-3.0, -3.0, -1.5], [3.0, 3.0, 1.5]).astype(np.float32)
+        bundle = _flag_bundle((10.0, 10.0, 10.0), tuple(float(a) for a in ang), fd=16, bs=0.5)
+        Mb = _bev_to_pvcs64(*(float(a) for a in bundle.angles))
+        S = vec(bundle.isocenter) + Mb @ np.array([0.0, -100.0, 0.0])
+        for ix in range(0, 16, 3):
+            for iy in range(0, 16, 3):
+                rc, out, flu = orc.beamlet_init(bundle, ix, iy, vol)
+                assert rc == 0
+                v, cases = _beamlet_geometry(bundle, ix, iy)
+                case = 1 if cases[1][2] > cases[2][2] else 2
+                c, proj, dist = cases[case]
+                Mc = _bev_to_pvcs64(*(float(a) for a in out.angles))
+                axis = Mc @ np.array([0.0, 1.0, 0.0])
+                # (theta, phi): the first of (t, p), (t, pi - p), (-t, p), (-t, pi - p) whose axis aligns
+                # with the bisector to cos >= 1 - 1e-4 (IMRTBeamBundle.cu:457-499)
+                d = Mb @ c
+                t = math.acos(d[1])
+                p = math.asin(d[2] / math.sqrt(d[2] ** 2 + d[0] ** 2 + 1e-8))
+                cand = [(t, p), (t, math.pi - p), (-t, p), (-t, math.pi - p)]
+                align = [float(np.dot(_bev_to_pvcs64(a, b, 0.0) @ np.array([0.0, 1.0, 0.0]), d)) for a, b in cand]
+                first = next(k for k in range(4) if align[k] >= 1 - 1e-4)
+                if all(abs(x - (1 - 1e-4)) > 2e-6 for x in align[:first + 1]):
+                    assert abs(float(out.angles[0]) - cand[first][0]) < 1e-4
+                    assert abs(math.remainder(float(out.angles[1]) - cand[first][1], 2 * math.pi)) < 1e-4
+                if _angle(axis, d) > 2e-5:
+                    # accepted within the reference's tolerance, not exactly aligned: the axis (and
+                    # with it the source) is off by up to acos(1 - 1e-4)
+                    assert _angle(axis, d) < math.acos(1 - 1e-4) + 1e-5
+                    seen["tolerance"] += 1
+                    continue
+                centre = Mb @ _n(v[0] + v[1] + v[2] + v[3])
+                assert _angle(axis, centre) < 1e-5
+                sad_c = float(out.sad)
+                assert abs(sad_c - 100.0 / c[1]) < 2e-5 * sad_c
+                Sc = vec(out.isocenter) + Mc @ np.array([0.0, -sad_c, 0.0])
+                assert np.linalg.norm(Sc - S) < 2.5e-3   # the axis tolerance (2e-5 rad) times ~100 cm
+                p0, p1 = _n(proj[0]), _n(proj[1])
+                codir = Mc @ np.array([0.0, 0.0, -1.0])
+                want = p0 if case == 1 else _n(p0 + p1)
+                if abs(cases[1][2] - cases[2][2]) > 1e-6 * cases[1][2]:
+                    err = _angle(codir, Mb @ want)
+                    if err > 2e-3:
+                        # the reference keeps +coll whenever it aligns to cos >= 1 - 1e-3
+                        # (IMRTBeamBundle.cu:519-533), so a small collimator angle may keep the
+                        # wrong sign: then the mirror (-coll) is the aligned one
+                        Mm = _bev_to_pvcs64(float(out.angles[0]), float(out.angles[1]), -float(out.angles[2]))
+                        assert _angle(Mm @ np.array([0.0, 0.0, -1.0]), Mb @ want) < 2e-3
+                        assert err < math.acos(1 - 1e-3) + 1e-4
+                        seen["mirror"] += 1
+                    seen[case] += 1
+                q0, q1 = c + dist * p0, c + dist * p1
+                fw, fh = float(out.beamlet_size[0]) * 4, float(out.beamlet_size[1]) * 4
+                assert abs(fw - np.linalg.norm(q0 - q1) * sad_c) < 1e-5 * fw
+                assert abs(fh - np.linalg.norm(q0 + q1 - 2 * c) * sad_c) < 1e-5 * fh
+                if case == 2 and abs(cases[1][2] - cases[2][2]) > 1e-6 * cases[1][2]:
+                    corners = [Mc @ np.array([sx * fw / 2, sad_c, sz * fh / 2]) for sx in (-1, 1) for sz in (-1, 1)]
+                    rect = [Mb @ (c + s * dist * p) for p in (p0, p1) for s in (-1, 1)]
+                    for k in corners:
+                        assert min(_angle(k, r) for r in rect) < 2e-5
+                on = np.zeros((16, 16), dtype=np.float32)
+                on[6:10, 6:10] = 1
+                assert (flu.reshape(16, 16) == on).all()
+                assert out.fmap_size[0] == 16 and out.long_spacing == np.float32(0.25)
+    assert seen[1] >= 10 and seen[2] >= 10, seen
