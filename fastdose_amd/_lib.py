@@ -136,6 +136,7 @@ SIGNATURES = {
     "fd_dosemat_copy_matrix": (C.c_int, [vp, C.c_uint64, P(C.c_uint64), P(C.c_uint64), fp]),
     "fd_dosemat_tofile": (C.c_int, [vp, C.c_char_p]),
     "fd_dosemat_fromfile": (C.c_int, [vp, C.c_char_p]),
+    "fd_dosemat_merge_folders": (C.c_int, [C.POINTER(C.c_char_p), C.c_int, C.c_char_p]),
     "fd_write_fluence_map": (C.c_int, [C.c_char_p, u8p, C.c_uint64, C.c_uint64]),
     "fd_dosemat_pack": (C.c_int, [vp, C.c_int, vp]),
     "fd_dosemat_copy_packed": (C.c_int, [vp, fp, P(C.c_uint16)]),

@@ -33,7 +33,7 @@ ORACLE_DIR = ROOT / "oracle"
 EXTRA_FLAGS = {"kernels_cccs.hip": ["-mllvm", "-join-splitedges=true", "-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
 
 HIP_SOURCES = ["api.hip", "engine.hip", "serve.hip", "kernels_terma.hip", "kernels_cccs.hip", "kernels_resample.hip", "kernels_csr.hip", "kernels_flags.hip", "kernels_spmm.hip", "spmm.hip"]
-CXX_SOURCES = ["geometry.cpp"]
+CXX_SOURCES = ["geometry.cpp", "dosemat_files.cpp"]
 ARCH = os.environ.get("FASTDOSE_OFFLOAD_ARCH", "gfx950")
 
 

@@ -19,6 +19,7 @@
 #include <vector>
 #include <sys/stat.h>
 
+#include "dosemat_files.h"
 #include "engine.h"
 #include "fd_internal.h"
 #include "geometry.h"
@@ -91,17 +92,6 @@ bool write_file(const std::string& path, const void* data, size_t bytes) {
     std::ofstream f(path, std::ios::binary | std::ios::trunc);
     if (!f.is_open()) return false;
     if (bytes) f.write((const char*)data, (std::streamsize)bytes);
-    return (bool)f;
-}
-
-bool read_file(const std::string& path, std::vector<char>& out) {
-    std::ifstream f(path, std::ios::binary);
-    if (!f.is_open()) return false;
-    f.seekg(0, std::ios::end);
-    std::streamoff n = f.tellg();
-    f.seekg(0, std::ios::beg);
-    out.resize((size_t)n);
-    if (n) f.read(out.data(), n);
     return (bool)f;
 }
 
@@ -1006,43 +996,51 @@ int fd_dosemat_fromfile(fd_dosemat* m, const char* folder) {
         set_error("null argument");
         return FD_ERR_ARG;
     }
-    const std::string dir(folder);
-    std::vector<char> nnz_b, rows_b, oidx_b, off_b, col_b, val_b;
-    if (!read_file(dir + "/NonZeroElements.bin", nnz_b) || !read_file(dir + "/numRowsPerMat.bin", rows_b) ||
-        !read_file(dir + "/OffsetBufferIdx.bin", oidx_b) || !read_file(dir + "/offsetsBuffer.bin", off_b) ||
-        !read_file(dir + "/columnsBuffer.bin", col_b) || !read_file(dir + "/valuesBuffer.bin", val_b)) {
-        set_error("cannot read the dose-matrix folder " + dir);
-        return FD_ERR_IO;
+    // Everything is read and checked on the host before the ensemble or the device is touched: a
+    // folder written for another volume, or a truncated one, must not become an out-of-range index
+    // in a kernel (the D^T X transpose counts columns with atomics, the copies follow the offsets).
+    FolderCSR f;
+    FD_TRY(read_dosemat_folder(folder, m->n_cols, false, f));
+    const uint64_t n_off = f.n_off, n_nnz = f.n_nnz;
+    const std::vector<char>& off_b = f.off_b;
+    const std::vector<char>& col_b = f.col_b;
+    const std::vector<char>& val_b = f.val_b;
+    fdi::DevBuf<uint64_t> offsets, columns;
+    fdi::DevBuf<float> values;
+    auto upload = [&]() -> int {
+        FD_TRY(offsets.reserve(std::max<uint64_t>(n_off, 1)));
+        FD_TRY(columns.reserve(std::max<uint64_t>(n_nnz, 1)));
+        FD_TRY(values.reserve(std::max<uint64_t>(n_nnz, 1)));
+        if (n_off) FD_HIP(hipMemcpy(offsets.ptr, off_b.data(), off_b.size(), hipMemcpyHostToDevice));
+        if (n_nnz) {
+            FD_HIP(hipMemcpy(columns.ptr, col_b.data(), col_b.size(), hipMemcpyHostToDevice));
+            FD_HIP(hipMemcpy(values.ptr, val_b.data(), val_b.size(), hipMemcpyHostToDevice));
+        }
+        return FD_OK;
+    };
+    const int rc = upload();
+    if (rc != FD_OK) {
+        offsets.release();
+        columns.release();
+        values.release();
+        return rc;
     }
-    const size_t nm = nnz_b.size() / 8;
-    if (rows_b.size() / 8 != nm || oidx_b.size() / 8 != nm || val_b.size() / 4 != col_b.size() / 8) {
-        set_error("inconsistent dose-matrix files under " + dir);
-        return FD_ERR_IO;
-    }
-    m->rows.assign((const uint64_t*)rows_b.data(), (const uint64_t*)rows_b.data() + nm);
-    m->nnz.assign((const uint64_t*)nnz_b.data(), (const uint64_t*)nnz_b.data() + nm);
-    m->off_off.assign((const uint64_t*)oidx_b.data(), (const uint64_t*)oidx_b.data() + nm);
-    m->nnz_off.resize(nm);
-    uint64_t acc = 0;
-    for (size_t i = 0; i < nm; i++) {
-        m->nnz_off[i] = acc;
-        acc += m->nnz[i];
-    }
-    m->used_off = off_b.size() / 8;
-    m->used_nnz = col_b.size() / 8;
+    // swap the new contents in only now
+    std::swap(m->offsets, offsets);
+    std::swap(m->columns, columns);
+    std::swap(m->values, values);
+    offsets.release();
+    columns.release();
+    values.release();
+    m->values16.release();
+    m->row_scale.release();
+    m->rows.swap(f.rows);
+    m->nnz.swap(f.nnz);
+    m->off_off.swap(f.off_off);
+    m->nnz_off.swap(f.nnz_off);
+    m->used_off = n_off;
+    m->used_nnz = n_nnz;
     m->packed = -1;
-    if (acc != m->used_nnz) {
-        set_error("NonZeroElements does not add up to the columns buffer");
-        return FD_ERR_IO;
-    }
-    FD_TRY(m->offsets.reserve(std::max<uint64_t>(m->used_off, 1)));
-    FD_TRY(m->columns.reserve(std::max<uint64_t>(m->used_nnz, 1)));
-    FD_TRY(m->values.reserve(std::max<uint64_t>(m->used_nnz, 1)));
-    if (m->used_off) FD_HIP(hipMemcpy(m->offsets.ptr, off_b.data(), off_b.size(), hipMemcpyHostToDevice));
-    if (m->used_nnz) {
-        FD_HIP(hipMemcpy(m->columns.ptr, col_b.data(), col_b.size(), hipMemcpyHostToDevice));
-        FD_HIP(hipMemcpy(m->values.ptr, val_b.data(), val_b.size(), hipMemcpyHostToDevice));
-    }
     return FD_OK;
 }
 

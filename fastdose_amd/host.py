@@ -331,6 +331,14 @@ def write_fluence_map(path, flags_per_beam) -> None:
     call("fd_write_fluence_map", str(path).encode(), _u8(arr), arr.shape[0], arr.shape[1])
 
 
+def merge_dosemat_folders(folders, out_folder) -> None:
+    """doseMatMerge (scripts/doseMerge.py:9-63): the doseMatFolder file sets of `folders`, in order,
+    concatenated into `out_folder` (fd_dosemat_merge_folders, host only)."""
+    names = [str(f).encode() for f in folders]
+    arr = (C.c_char_p * len(names))(*names)
+    call("fd_dosemat_merge_folders", arr, len(names), str(out_folder).encode())
+
+
 def read_dosemat_folder(folder):
     """readDose.m semantics (scripts/readDose.m:14-53): list of per-beam CSR (offsets, cols, vals)."""
     folder = Path(folder)

@@ -282,8 +282,19 @@ int fd_dosemat_copy_matrix(const fd_dosemat* m, uint64_t k, uint64_t* offsets, u
 /* MatCSREnsemble::tofile (IMRTSparseMatEns.cu:168-239): the doseMatFolder layout read by
  * scripts/readDose.m. */
 int fd_dosemat_tofile(const fd_dosemat* m, const char* folder);
-/* MatCSREnsemble::fromfile (IMRTSparseMatEns.cu:242-336). */
+/* MatCSREnsemble::fromfile (IMRTSparseMatEns.cu:242-336).  The folder is read and checked on the
+ * host first (table sizes, every matrix's offsets inside offsetsBuffer, starting at 0, non-decreasing
+ * and ending at its nonzero count, every column below the ensemble's n_cols); FD_ERR_IO leaves the
+ * ensemble as it was.  Replaces the ensemble's contents. */
 int fd_dosemat_fromfile(fd_dosemat* m, const char* folder);
+
+/* scripts/doseMerge.py:9-63 (doseMatMerge): concatenates the doseMatFolder file sets of n_folders
+ * folders, in order, into out_folder -- NonZeroElements, numRowsPerMat, offsetsBuffer,
+ * columnsBuffer, valuesBuffer and fluenceMap.bin byte for byte as that script writes them -- and
+ * also writes CumuNonZeroElements, CumuNumRowsPerMat and OffsetBufferIdx of the merged set (which
+ * the script leaves out), so fd_dosemat_fromfile reads the result.  Every input is checked as
+ * fd_dosemat_fromfile checks it before anything is written.  Host only (no device). */
+int fd_dosemat_merge_folders(const char* const* folders, int n_folders, const char* out_folder);
 
 /* 16-bit copy of the ensemble's values: per row (beamlet, all matrices in order) an FP32 scale =
  * the row maximum, and q = value / scale rounded to nearest even in BF16 or FP16.  After it,

@@ -230,6 +230,30 @@ def test_dosemat_files_roundtrip(torch_cuda, tmp_path):
         for k in range(2):
             a, b = dm.matrix(k), dm2.matrix(k)
             assert all(np.array_equal(x, y) for x, y in zip(a, b))
+        # a truncated folder is refused on the host and the loaded ensemble stays as it was
+        bad = tmp_path / "truncated"
+        bad.mkdir()
+        for f in folder.iterdir():
+            (bad / f.name).write_bytes(f.read_bytes())
+        (bad / "columnsBuffer.bin").write_bytes((folder / "columnsBuffer.bin").read_bytes()[:-8])
+        with pytest.raises(fd.FastDoseError):
+            dm2.fromfile(bad)
+        assert dm2.info() == dm.info()
+        assert all(np.array_equal(x, y) for x, y in zip(dm.matrix(1), dm2.matrix(1)))
+        # doseMerge.py: two runs' folders merged read back as their matrices in order
+        dm_b = fd.dose_mat_construction(ctx, cfg.bundles[1:], cfg.flags[1:], cfg.extent)
+        folder_b = tmp_path / "second" / "doseMatFolder"
+        folder_b.parent.mkdir()
+        dm_b.tofile(folder_b)
+        fd.write_fluence_map(folder / "fluenceMap.bin", cfg.flags)
+        fd.write_fluence_map(folder_b / "fluenceMap.bin", cfg.flags[1:])
+        merged = tmp_path / "merged"
+        fd.merge_dosemat_folders([folder, folder_b], merged)
+        dm3 = fd.DoseMat(dm.n_cols)
+        dm3.fromfile(merged)
+        assert dm3.info()[0] == 3
+        for k, (src, j) in enumerate(((dm, 0), (dm, 1), (dm_b, 0))):
+            assert all(np.array_equal(x, y) for x, y in zip(src.matrix(j), dm3.matrix(k))), k
         # the dose matrix rows equal the split-stage rows
         beams, flu = fd.beamlets_init(cfg.bundles[0], cfg.flags[0], cfg.volume)
         g = gpu_stages(ctx, beams, flu, cfg.extent)
