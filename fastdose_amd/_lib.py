@@ -125,6 +125,7 @@ SIGNATURES = {
     "fd_cccs": (C.c_int, [vp, P(Beam), C.c_int, vp, vp, vp, C.c_uint64, vp]),
     "fd_resample_csr": (C.c_int, [vp, P(Beam), C.c_int, vp, C.c_uint64, C.c_float, P(CSR), vp]),
     "fd_dense_to_csr": (C.c_int, [vp, C.c_uint64, C.c_uint64, P(CSR), vp]),
+    "fd_dense_to_csr_noalloc": (C.c_int, [vp, C.c_uint64, C.c_uint64, P(CSR), vp]),
     "fd_cccs_segments": (C.c_int, [vp, P(Beam), C.c_int, P(C.c_int64)]),
     "fd_dosemat_create": (C.c_int, [C.c_uint64, C.c_uint64, P(vp)]),
     "fd_dosemat_destroy": (C.c_int, [vp]),

@@ -32,7 +32,7 @@ ORACLE_DIR = ROOT / "oracle"
 # -> 1.46 ms on the 128^3 slab beam).
 EXTRA_FLAGS = {"kernels_cccs.hip": ["-mllvm", "-join-splitedges=true", "-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
 
-HIP_SOURCES = ["api.hip", "engine.hip", "serve.hip", "kernels_terma.hip", "kernels_cccs.hip", "kernels_resample.hip", "kernels_csr.hip", "kernels_flags.hip", "kernels_spmm.hip", "spmm.hip"]
+HIP_SOURCES = ["api.hip", "engine.hip", "serve.hip", "noalloc.hip", "kernels_terma.hip", "kernels_cccs.hip", "kernels_resample.hip", "kernels_csr.hip", "kernels_flags.hip", "kernels_spmm.hip", "spmm.hip"]
 CXX_SOURCES = ["geometry.cpp", "dosemat_files.cpp"]
 ARCH = os.environ.get("FASTDOSE_OFFLOAD_ARCH", "gfx950")
 

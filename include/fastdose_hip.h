@@ -230,6 +230,16 @@ int fd_resample_csr(fd_ctx* ctx, const fd_beam* beams, int n, const float* dose_
  * row-major dense (dev) -> CSR.  Same two-phase capacity rule as fd_resample_csr. */
 int fd_dense_to_csr(const float* dense_dev, uint64_t rows, uint64_t cols, fd_csr* out, void* stream);
 
+/* fd_dense_to_csr without any allocation: no counterpart in the reference.  For callers that
+ * cannot allocate device or pinned memory (tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and two-phase capacity rule as fd_dense_to_csr; its per-row counts and
+ * row offsets live in a library-held device array of about 12 MiB per device (held until the
+ * process ends), so rows is limited to FD_NOALLOC_MAX_ROWS (FD_ERR_ARG beyond).  Every call on
+ * one device shares that array: calls that overlap in time on one device race.  The call
+ * synchronises `stream` before it returns. */
+#define FD_NOALLOC_MAX_ROWS (1u << 20)
+int fd_dense_to_csr_noalloc(const float* dense_dev, uint64_t rows, uint64_t cols, fd_csr* out, void* stream);
+
 /* Number of (ray, segment) steps of the CCCS march of each beamlet (host count of the same FP32
  * lattice the kernel walks); used for the FLOP accounting and load balancing. */
 int fd_cccs_segments(fd_ctx* ctx, const fd_beam* beams, int n, int64_t* nseg_out);
