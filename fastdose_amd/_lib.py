@@ -120,6 +120,7 @@ SIGNATURES = {
     "fd_ctx_set_options": (C.c_int, [vp, C.c_int]),
     "fd_ctx_density": (C.c_int, [vp, P(vp)]),
     "fd_ctx_timings": (C.c_int, [vp, fp, C.c_int]),
+    "fd_ctx_timing_launches": (C.c_int, [vp, P(C.c_int), C.c_int]),
     "fd_ctx_set_threshold": (C.c_int, [vp, C.c_float]),
     "fd_terma": (C.c_int, [vp, P(Beam), C.c_int, vp, vp, vp, C.c_uint64, vp]),
     "fd_cccs": (C.c_int, [vp, P(Beam), C.c_int, vp, vp, vp, C.c_uint64, vp]),

@@ -472,17 +472,20 @@ int fd_ctx_timings(fd_ctx* ctx, float* ms, int n) {
         set_error("null argument");
         return FD_ERR_ARG;
     }
-    for (int k = 0; k < n; k++) {
-        ms[k] = -1.0f;
-        const Workspace& w = ctx->ws;
-        if (k >= FD_T_COUNT || !w.ev_used[k] || !w.ev[0]) continue;
-        hipEvent_t a = (k == FD_T_CONE_SUM) ? w.ev[2 * FD_T_CCCS + 1] : w.ev[2 * k];
-        hipEvent_t b = w.ev[2 * k + 1];
-        FD_HIP(hipEventSynchronize(b));
-        float t = 0.0f;
-        FD_HIP(hipEventElapsedTime(&t, a, b));
-        ms[k] = t;
+    Workspace& w = ctx->ws;
+    for (int k = 0; k < FD_T_COUNT; k++) FD_TRY(w.fold(k));
+    for (int k = 0; k < n; k++) ms[k] = (k < FD_T_COUNT && w.ev_used[k]) ? (float)w.acc_ms[k] : -1.0f;
+    return FD_OK;
+}
+
+int fd_ctx_timing_launches(fd_ctx* ctx, int* launches, int n) {
+    if (!ctx || !launches || n < 0) {
+        set_error("null argument");
+        return FD_ERR_ARG;
     }
+    Workspace& w = ctx->ws;
+    for (int k = 0; k < FD_T_COUNT; k++) FD_TRY(w.fold(k));
+    for (int k = 0; k < n; k++) launches[k] = (k < FD_T_COUNT && w.ev_used[k]) ? w.acc_n[k] : 0;
     return FD_OK;
 }
 
@@ -498,6 +501,7 @@ int fd_ctx_density(fd_ctx* ctx, const float** density_dev) {
 int fd_terma(fd_ctx* ctx, const fd_beam* beams, int n, const float* fluence_dev, float* terma_dev,
              float* density_bev_dev, uint64_t stride, void* stream) {
     FD_TRY(check_ctx(ctx));
+    ctx->ws.call++;
     FD_TRY(check_beams(beams, n));
     FD_TRY(check_stride(beams, n, stride));
     if (n == 0) return FD_OK;
@@ -513,6 +517,7 @@ int fd_terma(fd_ctx* ctx, const fd_beam* beams, int n, const float* fluence_dev,
 int fd_cccs(fd_ctx* ctx, const fd_beam* beams, int n, const float* terma_dev, const float* density_bev_dev,
             float* dose_bev_dev, uint64_t stride, void* stream) {
     FD_TRY(check_ctx(ctx));
+    ctx->ws.call++;
     FD_TRY(check_beams(beams, n));
     FD_TRY(check_stride(beams, n, stride));
     if (n == 0) return FD_OK;
@@ -529,6 +534,7 @@ int fd_cccs(fd_ctx* ctx, const fd_beam* beams, int n, const float* terma_dev, co
 int fd_resample_csr(fd_ctx* ctx, const fd_beam* beams, int n, const float* dose_bev_dev, uint64_t stride,
                     float extent, fd_csr* out, void* stream) {
     FD_TRY(check_ctx(ctx));
+    ctx->ws.call++;
     FD_TRY(check_beams(beams, n));
     FD_TRY(check_stride(beams, n, stride));
     if (!out || !out->offsets || (n > 0 && !dose_bev_dev)) {
@@ -699,6 +705,7 @@ int fd_dosemat_destroy(fd_dosemat* m) {
 int fd_dosemat_add_beams(fd_ctx* ctx, fd_dosemat* m, const fd_beam* beamlets, const float* fluence,
                          const int* counts, int n_beams, int max_chunk, float extent, void* stream) {
     FD_TRY(check_ctx(ctx));
+    ctx->ws.call++;
     if (!m || n_beams < 0 || (n_beams > 0 && !counts)) {
         set_error("null argument");
         return FD_ERR_ARG;

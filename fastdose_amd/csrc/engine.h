@@ -108,9 +108,15 @@ struct Workspace {
     uint64_t* host_rowoff = nullptr;  // pinned copy of rowoff
     size_t host_rowoff_cap = 0;
     double cand_per_beamlet = 0.0;    // running estimate that sizes the staging up front
-    // kernel timers: slot k uses ev[2k] / ev[2k+1]
+    // kernel timers: slot k uses ev[2k] / ev[2k+1] (the cone sum starts at the CCCS end event);
+    // per slot, the milliseconds and launches summed over the last API call that launched it
     hipEvent_t ev[2 * FD_T_COUNT] = {};
     bool ev_used[FD_T_COUNT] = {};
+    bool ev_pending[FD_T_COUNT] = {};  // recorded, not yet added to acc_ms
+    double acc_ms[FD_T_COUNT] = {};
+    int acc_n[FD_T_COUNT] = {};
+    uint64_t acc_call[FD_T_COUNT] = {};
+    uint64_t call = 0;                 // API calls that launched work on this workspace
     bool timing = false;
     hipGraphExec_t gexec = nullptr;   // last instantiated batch graph (FD_OPT_GRAPH)
     hipStream_t own_stream = nullptr; // capture stream when the caller passes the null stream
@@ -119,8 +125,24 @@ struct Workspace {
     void release();
     void mark(int slot, int which, hipStream_t s) {
         if (!timing || !ev[0]) return;
+        if (which == 0) fold(slot);  // the previous launch's pair is about to be re-recorded
         (void)hipEventRecord(ev[2 * slot + which], s);
-        if (which == 1) ev_used[slot] = true;
+        if (which == 1) end_mark(slot);
+    }
+    // The end of slot's launch was recorded (by mark or by a launcher that records its own events).
+    void end_mark(int slot) {
+        ev_used[slot] = true;
+        ev_pending[slot] = true;
+        if (acc_call[slot] != call) {
+            acc_call[slot] = call;
+            acc_ms[slot] = 0.0;
+            acc_n[slot] = 0;
+        }
+    }
+    // Adds the recorded launch of slot (if any) to its sums; waits for its end event.
+    int fold(int slot);
+    void fold_all() {
+        for (int k = 0; k < FD_T_COUNT; k++) (void)fold(k);
     }
     // Device bytes this workspace holds.
     size_t footprint() const;

@@ -61,6 +61,20 @@ void Workspace::release() {
     scan_tmp.release();
 }
 
+int Workspace::fold(int slot) {
+    if (!ev_pending[slot]) return FD_OK;
+    // the cone sum's launch starts where the CCCS kernel's ends
+    hipEvent_t a = (slot == FD_T_CONE_SUM) ? ev[2 * FD_T_CCCS + 1] : ev[2 * slot];
+    hipEvent_t b = ev[2 * slot + 1];
+    ev_pending[slot] = false;
+    FD_HIP(hipEventSynchronize(b));
+    float t = 0.0f;
+    FD_HIP(hipEventElapsedTime(&t, a, b));
+    acc_ms[slot] += t;
+    acc_n[slot] += 1;
+    return FD_OK;
+}
+
 size_t Workspace::footprint() const {
     return desc.bytes() + partial.bytes() + fluence.bytes() + terma.bytes() + dens_bev.bytes() + dose_bev.bytes() +
            cand_count.bytes() + nnz_count.bytes() + cand_off.bytes() + nnz_off.bytes() + tmp_cols.bytes() +
@@ -233,6 +247,8 @@ int run_cccs(const fd_ctx* ctx, Workspace& w, int options, int n, int max_L, con
     hipEvent_t evs[3];
     const bool t = w.timing && w.ev[0];
     if (t) {
+        FD_TRY(w.fold(FD_T_CCCS));  // before their events are recorded again
+        FD_TRY(w.fold(FD_T_CONE_SUM));
         evs[0] = w.ev[2 * FD_T_CCCS];
         evs[1] = w.ev[2 * FD_T_CCCS + 1];
         evs[2] = w.ev[2 * FD_T_CONE_SUM + 1];
@@ -242,8 +258,8 @@ int run_cccs(const fd_ctx* ctx, Workspace& w, int options, int n, int max_L, con
                                t ? evs : nullptr);
     if (t) {
         // the cone sum starts where the CCCS kernel stops (timings pair them)
-        w.ev_used[FD_T_CCCS] = true;
-        w.ev_used[FD_T_CONE_SUM] = true;
+        w.end_mark(FD_T_CCCS);
+        w.end_mark(FD_T_CONE_SUM);
     }
     if (e != hipSuccess) {
         set_error(std::string("cccs launch: ") + hipGetErrorString(e));
@@ -483,6 +499,7 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
         }
     }
     update_candidate_estimate(w, ratio);
+    w.fold_all();
     out->nnz = w.host_totals[1];
     out->rowoff.assign(w.host_rowoff, w.host_rowoff + n + 1);
     return FD_OK;

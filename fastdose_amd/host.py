@@ -154,11 +154,19 @@ class Context:
         return {f: getattr(t, f) for f, _ in t._fields_}
 
     def timings(self) -> dict:
-        """Per-kernel milliseconds of the last launches (needs FD_OPT_TIMING)."""
+        """Per-kernel milliseconds of the last call that launched each kernel, summed over its
+        launches (needs FD_OPT_TIMING)."""
         n = len(_lib.TIMING_SLOTS)
         ms = (C.c_float * n)()
         call("fd_ctx_timings", self._h, ms, n)
         return {k: float(ms[i]) for i, k in enumerate(_lib.TIMING_SLOTS) if ms[i] >= 0}
+
+    def timing_launches(self) -> dict:
+        """How many launches each kernel's timings() entry sums."""
+        n = len(_lib.TIMING_SLOTS)
+        cnt = (C.c_int * n)()
+        call("fd_ctx_timing_launches", self._h, cnt, n)
+        return {k: int(cnt[i]) for i, k in enumerate(_lib.TIMING_SLOTS) if cnt[i] > 0}
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:

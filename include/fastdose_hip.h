@@ -202,9 +202,12 @@ int fd_ctx_set_options(fd_ctx* ctx, int options);
  * (IMRTSparseMatEns.cu:108-161) does; the reference has no threshold. */
 int fd_ctx_set_threshold(fd_ctx* ctx, float rel);
 /* Milliseconds each kernel slot took in the last call that launched it (FD_OPT_TIMING set),
- * measured with HIP events on the launch stream; -1 for slots not run.  Plays the role of the
- * reference's cudaEvent timers (TermaComp.cu:310-333, DoseComp.cu:449-472). */
+ * measured with HIP events on the launch stream and summed over that call's launches (a batched
+ * call launches each kernel once per chunk; graph replay records no events); -1 for slots not run.
+ * Plays the role of the reference's cudaEvent timers (TermaComp.cu:310-333, DoseComp.cu:449-472). */
 int fd_ctx_timings(fd_ctx* ctx, float* ms, int n);
+/* How many launches the sums of fd_ctx_timings cover, per slot (0 for slots not run). */
+int fd_ctx_timing_launches(fd_ctx* ctx, int* launches, int n);
 /* Device pointer of the resident density volume (x fastest). */
 int fd_ctx_density(fd_ctx* ctx, const float** density_dev);
 

@@ -1,7 +1,9 @@
 """Shared helpers of the GPU tests: run the HIP stages through the C-ABI and the oracle beside them."""
 from __future__ import annotations
 
+import os
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -64,6 +66,41 @@ def oracle_beamlet(cfg, beam, flu, flags=0):
     return orc.beamlet_pipeline(orc.cast(orc.Beam, beam), flu, orc.cast(orc.Volume, cfg.volume), cfg.density,
                                 orc.cast(orc.Spectrum, cfg.spectrum), orc.cast(orc.Kernel, cfg.kernel), cfg.extent,
                                 flags)
+
+
+def oracle_many(cfg, items, flags=0):
+    """beamlet_pipeline of the FP64 oracle for each (beam, fluence) of items, on the host cores (the
+    oracle's ctypes calls release the interpreter lock)."""
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=max(1, min(16, threads))) as ex:
+        return list(ex.map(lambda it: oracle_beamlet(cfg, it[0], it[1], flags), items))
+
+
+def check_row(vol, cols, vals, ref, tol=0.005, tag=""):
+    """One CSR row (a beamlet's dose column) against the oracle's: |D - D_fp64| <= tol x the beamlet's
+    maximum over the union of columns (missing = 0), gamma 1 %/1 mm >= 99 %, ascending columns, and
+    columns present on one side only carry at most 1e-6 of the maximum.  Returns the relative error."""
+    cols = np.asarray(cols).astype(np.int64)
+    vals = np.asarray(vals)
+    assert np.all(np.diff(cols) > 0), (tag, "columns not ascending")
+    err, sym = dense_diff(cols, vals, ref["cols"], ref["vals"])
+    err /= max(float(ref["vals"].max()), 1e-30)
+    assert err <= tol, (tag, err)
+    gp = gamma_pass_rate(vol, ref["cols"], ref["vals"], cols, vals, dd=0.01, dta=0.1)
+    assert gp >= 0.99, (tag, gp)
+    if len(sym):
+        tiny = 1e-6 * ref["vals"].max()
+        vg = dict(zip(cols.tolist(), vals.tolist()))
+        vr = dict(zip(ref["cols"].tolist(), ref["vals"].tolist()))
+        for c in sym.tolist():
+            assert max(vg.get(c, 0.0), vr.get(c, 0.0)) <= tiny, (tag, c)
+        assert len(sym) <= 1e-4 * len(np.union1d(cols, ref["cols"])) + 2, tag
+    return err
+
+
+def same_matrix(a, b):
+    """Two host CSR matrices (offsets, columns, values) equal bit for bit."""
+    return all(x.dtype == y.dtype and np.array_equal(x, y) for x, y in zip(a, b))
 
 
 def row(g, i):

@@ -122,7 +122,8 @@ def test_config2_sampled_beamlets(torch_cuda):
     # size-independent properties on every row
     off = g["offsets"]
     assert off[0] == 0 and np.all(np.diff(off) > 0)
-    check_against_oracle(cfg, beams, flu, g, [0, 37, 120, 135, 255], stage_isolated=True)
+    check_against_oracle(cfg, beams, flu, g, [0, 17, 37, 60, 85, 120, 135, 150, 178, 200, 230, 255],
+                         stage_isolated=True)
 
 
 def test_segment_count_matches_oracle(torch_cuda):
