@@ -6,7 +6,10 @@
 // and a workspace; when free it takes the next beams (whole beams, up to max_batch_beamlets),
 // admits the batch if the scratch of everything in flight stays within the HBM budget (a batch
 // is always admitted when nothing else is in flight), runs the whole path with run_batch, and
-// splits the batch's CSR rows back into per-beam device matrices.  Beams that arrive while
+// hands the batch's CSR rows to its beams as one pooled block (the workspace's output buffers are
+// swapped into it, nothing is copied or allocated); fd_serve_fetch copies a request's rows into the
+// caller's ensemble on the server's fetch stream and returns the block to the pool once every beam
+// of its batch has been fetched.  Beams that arrive while
 // others run join the next batch of whichever slot frees first, so their work overlaps on the
 // device with the batches already running.
 #include <hip/hip_runtime.h>
@@ -29,11 +32,19 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// The CSR rows of one finished batch.  The slot hands its workspace's output buffers over to the
+// block (a swap, no copy) and takes the block's previous buffers in exchange; the block goes back to
+// the server's pool when every beam of the batch has been fetched.
+struct Block {
+    DevBuf<uint64_t> cols;
+    DevBuf<float> vals;
+    std::vector<uint64_t> rowoff;  // n + 1 row offsets into cols / vals (host)
+    int beams_left = 0;            // beams of the batch not fetched yet
+};
+
 struct BeamOut {
-    uint64_t rows = 0, nnz = 0;
-    uint64_t* offsets = nullptr;  // device, rows + 1
-    uint64_t* columns = nullptr;  // device, nnz
-    float* values = nullptr;      // device, nnz
+    Block* block = nullptr;
+    uint64_t first_row = 0, rows = 0;
     bool done = false;
 };
 
@@ -74,6 +85,9 @@ struct fd_server {
     std::vector<std::thread> threads;
     std::vector<Workspace> ws;
     std::vector<hipStream_t> streams;
+    hipStream_t fetch_stream = nullptr;        // copies of fetched beams into callers' ensembles
+    std::vector<std::unique_ptr<Block>> blocks;  // every block, for release at stop
+    std::vector<Block*> free_blocks;
     Clock::time_point t0;
     // statistics
     std::vector<double> lat_ms;
@@ -88,29 +102,22 @@ double since(const fd_server* s, Clock::time_point t) {
     return std::chrono::duration<double>(t - s->t0).count();
 }
 
-void free_beam(BeamOut& b) {
-    if (b.offsets) (void)hipFree(b.offsets);
-    if (b.columns) (void)hipFree(b.columns);
-    if (b.values) (void)hipFree(b.values);
-    b.offsets = b.columns = nullptr;
-    b.values = nullptr;
+// A block whose beams have all been fetched goes back to the pool (server lock held).
+void release_beam(fd_server* s, BeamOut& o) {
+    if (!o.block) return;
+    if (--o.block->beams_left == 0) s->free_blocks.push_back(o.block);
+    o.block = nullptr;
 }
 
-// Device copies of one beam's rows of a finished batch (offsets re-based to the beam's first row).
-int split_beam(Workspace& w, const BatchResult& res, uint64_t first_row, uint64_t rows, BeamOut& o,
-               std::vector<uint64_t>& host_off, hipStream_t st) {
-    const uint64_t base = res.rowoff[first_row];
-    o.rows = rows;
-    o.nnz = res.rowoff[first_row + rows] - base;
-    FD_HIP(hipMalloc((void**)&o.offsets, (rows + 1) * sizeof(uint64_t)));
-    if (o.nnz) {
-        FD_HIP(hipMalloc((void**)&o.columns, o.nnz * sizeof(uint64_t)));
-        FD_HIP(hipMalloc((void**)&o.values, o.nnz * sizeof(float)));
-        FD_HIP(hipMemcpyAsync(o.columns, w.out_cols.ptr + base, o.nnz * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
-        FD_HIP(hipMemcpyAsync(o.values, w.out_vals.ptr + base, o.nnz * sizeof(float), hipMemcpyDeviceToDevice, st));
+// A block from the pool, or a new one (server lock held).
+Block* take_block(fd_server* s) {
+    if (!s->free_blocks.empty()) {
+        Block* b = s->free_blocks.back();
+        s->free_blocks.pop_back();
+        return b;
     }
-    for (uint64_t j = 0; j <= rows; j++) host_off.push_back(res.rowoff[first_row + j] - base);
-    return FD_OK;
+    s->blocks.push_back(std::make_unique<Block>());
+    return s->blocks.back().get();
 }
 
 void slot_loop(fd_server* s, int slot) {
@@ -164,56 +171,42 @@ void slot_loop(fd_server* s, int slot) {
         }
         int rc = run_batch(s->ctx, w, s->opts.options, beams.data(), (int)beams.size(), flu.data(), s->opts.extent,
                            s->opts.threshold, st, &res);
-        std::vector<BeamOut> outs(batch.size());
-        std::vector<uint64_t> host_off;
-        uint64_t row = 0;
-        for (size_t k = 0; k < batch.size() && rc == FD_OK; k++) {
-            const uint64_t c = (uint64_t)batch[k].req->counts[batch[k].beam];
-            rc = split_beam(w, res, row, c, outs[k], host_off, st);
-            row += c;
-        }
-        if (rc == FD_OK) {
-            size_t h = 0;
-            for (size_t k = 0; k < batch.size() && rc == FD_OK; k++) {
-                hipError_t e = hipMemcpyAsync(outs[k].offsets, host_off.data() + h, (outs[k].rows + 1) * sizeof(uint64_t),
-                                              hipMemcpyHostToDevice, st);
-                if (e != hipSuccess) {
-                    set_error(std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
-                    rc = FD_ERR_HIP;
-                }
-                h += outs[k].rows + 1;
-            }
-        }
-        if (rc == FD_OK) {
-            hipError_t e = hipStreamSynchronize(st);
-            if (e != hipSuccess) {
-                set_error(std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
-                rc = FD_ERR_HIP;
-            }
-        }
+        // run_batch returns with the batch finished on the device
         const std::string err = rc == FD_OK ? std::string() : std::string(fd_last_error());
         const Clock::time_point t_done = Clock::now();
         {
             std::lock_guard<std::mutex> lk(s->mu);
+            Block* blk = nullptr;
+            if (rc == FD_OK) {
+                blk = take_block(s);
+                std::swap(blk->cols, w.out_cols);
+                std::swap(blk->vals, w.out_vals);
+                blk->rowoff.swap(res.rowoff);
+                blk->beams_left = (int)batch.size();
+            }
             s->inflight_bytes -= bytes;
             s->inflight_batches--;
             s->batches++;
+            uint64_t row = 0;
             for (size_t k = 0; k < batch.size(); k++) {
                 Request& q = *batch[k].req;
                 BeamOut& o = q.out[batch[k].beam];
+                const uint64_t c = (uint64_t)q.counts[batch[k].beam];
                 if (rc == FD_OK) {
-                    o = outs[k];
+                    o.block = blk;
+                    o.first_row = row;
+                    o.rows = c;
                 } else {
-                    free_beam(outs[k]);
                     q.failed = true;
                     q.error = err;
                 }
+                row += c;
                 o.done = true;
                 q.beams_left--;
                 const double ms = std::chrono::duration<double, std::milli>(t_done - q.t_submit).count();
                 s->lat_ms.push_back(ms);
                 s->beams_done++;
-                s->beamlets_done += (uint64_t)q.counts[batch[k].beam];
+                s->beamlets_done += c;
                 if (q.beams_left == 0) s->requests_done++;
             }
             s->last_done = since(s, t_done);
@@ -272,6 +265,18 @@ int fd_serve_start(fd_ctx* ctx, const fd_serve_opts* opts, fd_server** out) {
             }
             delete s;
             return r != FD_OK ? r : FD_ERR_HIP;
+        }
+    }
+    {
+        hipError_t e = hipStreamCreateWithFlags(&s->fetch_stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+            for (int j = 0; j < o.n_slots; j++) {
+                s->ws[j].release();
+                (void)hipStreamDestroy(s->streams[j]);
+            }
+            delete s;
+            return FD_ERR_HIP;
         }
     }
     for (int i = 0; i < o.n_slots; i++) s->threads.emplace_back(slot_loop, s, i);
@@ -392,21 +397,45 @@ int fd_serve_fetch(fd_server* s, uint64_t id, fd_dosemat* m) {
         set_error("dose matrix column count differs from the context's voxel count");
         rc = FD_ERR_ARG;
     }
+    // every beam's offsets (re-based to its first row) in one host array that outlives the copies
+    std::vector<uint64_t> host_off;
+    if (rc == FD_OK) {
+        size_t total = 0;
+        for (const BeamOut& o : q->out) total += o.rows + 1;
+        host_off.reserve(total);
+    }
+    hipStream_t fs = s->fetch_stream;
     for (size_t k = 0; k < q->out.size() && rc == FD_OK; k++) {
-        BeamOut& o = q->out[k];
+        const BeamOut& o = q->out[k];
+        const std::vector<uint64_t>& ro = o.block->rowoff;
+        const uint64_t base = ro[o.first_row], nnz = ro[o.first_row + o.rows] - base;
         uint64_t *od, *cd;
         float* vd;
-        rc = append_matrix(m, o.rows, o.nnz, nullptr, &od, &cd, &vd);
+        rc = append_matrix(m, o.rows, nnz, fs, &od, &cd, &vd);
         if (rc != FD_OK) break;
-        hipError_t e = hipMemcpy(od, o.offsets, (o.rows + 1) * sizeof(uint64_t), hipMemcpyDeviceToDevice);
-        if (e == hipSuccess && o.nnz) e = hipMemcpy(cd, o.columns, o.nnz * sizeof(uint64_t), hipMemcpyDeviceToDevice);
-        if (e == hipSuccess && o.nnz) e = hipMemcpy(vd, o.values, o.nnz * sizeof(float), hipMemcpyDeviceToDevice);
+        const size_t h = host_off.size();
+        for (uint64_t j = 0; j <= o.rows; j++) host_off.push_back(ro[o.first_row + j] - base);
+        hipError_t e = hipMemcpyAsync(od, host_off.data() + h, (o.rows + 1) * sizeof(uint64_t),
+                                      hipMemcpyHostToDevice, fs);
+        if (e == hipSuccess && nnz)
+            e = hipMemcpyAsync(cd, o.block->cols.ptr + base, nnz * sizeof(uint64_t), hipMemcpyDeviceToDevice, fs);
+        if (e == hipSuccess && nnz)
+            e = hipMemcpyAsync(vd, o.block->vals.ptr + base, nnz * sizeof(float), hipMemcpyDeviceToDevice, fs);
         if (e != hipSuccess) {
-            set_error(std::string("hipMemcpy: ") + hipGetErrorString(e));
+            set_error(std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
             rc = FD_ERR_HIP;
         }
     }
-    for (BeamOut& o : q->out) free_beam(o);
+    // the blocks go back to the pool only once the copies out of them have finished
+    const hipError_t es = hipStreamSynchronize(fs);
+    if (es != hipSuccess && rc == FD_OK) {
+        set_error(std::string("hipStreamSynchronize: ") + hipGetErrorString(es));
+        rc = FD_ERR_HIP;
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (BeamOut& o : q->out) release_beam(s, o);
+    }
     return rc;
 }
 
@@ -466,12 +495,19 @@ int fd_serve_stop(fd_server* s) {
     s->cv_work.notify_all();
     for (auto& t : s->threads) t.join();
     (void)hipSetDevice(s->ctx->device);
-    for (auto& kv : s->requests)
-        for (BeamOut& o : kv.second->out) free_beam(o);
     for (size_t i = 0; i < s->ws.size(); i++) {
         if (s->streams[i]) (void)hipStreamSynchronize(s->streams[i]);
         s->ws[i].release();
         if (s->streams[i]) (void)hipStreamDestroy(s->streams[i]);
+    }
+    if (s->fetch_stream) {
+        (void)hipStreamSynchronize(s->fetch_stream);
+        (void)hipStreamDestroy(s->fetch_stream);
+    }
+    // every block, including those of requests never fetched
+    for (auto& b : s->blocks) {
+        b->cols.release();
+        b->vals.release();
     }
     delete s;
     return FD_OK;
