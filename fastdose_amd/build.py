@@ -30,7 +30,8 @@ ORACLE_DIR = ROOT / "oracle"
 # more of the step loop's X states in one register set across the rotation branches, and the ILP
 # scheduling strategy interleaves the independent per-slot chains better (CCCS kernel 1.52 -> 1.48
 # -> 1.46 ms on the 128^3 slab beam).
-EXTRA_FLAGS = {"kernels_cccs.hip": ["-mllvm", "-join-splitedges=true", "-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
+EXTRA_FLAGS = {"kernels_cccs.hip": ["-mllvm", "-join-splitedges=true", "-mllvm", "-amdgpu-sched-strategy=max-ilp",
+                                    "-mllvm", "-structurizecfg-skip-uniform-regions=true"]}
 
 HIP_SOURCES = ["api.hip", "engine.hip", "serve.hip", "noalloc.hip", "kernels_terma.hip", "kernels_cccs.hip", "kernels_resample.hip", "kernels_csr.hip", "kernels_flags.hip", "kernels_spmm.hip", "spmm.hip"]
 CXX_SOURCES = ["geometry.cpp", "dosemat_files.cpp"]

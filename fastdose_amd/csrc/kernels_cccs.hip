@@ -55,10 +55,17 @@ __device__ __forceinline__ float uniform(float v) {
 
 // value of the neighbouring lane in the same 16-lane row (DPP row rotation)
 __device__ __forceinline__ float row_from_left(float v) {   // dst[i] = src[i - 1]
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x121, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x121, 0xF, 0xF, false));
 }
 __device__ __forceinline__ float row_from_right(float v) {  // dst[i] = src[i + 1]
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x12F, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x12F, 0xF, 0xF, false));
+}
+// the same shifts without the wrap: the lane that would receive the other end of its row gets 0
+__device__ __forceinline__ float row_from_left_zero(float v) {   // dst[i] = i % 16 ? src[i - 1] : 0
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x111, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float row_from_right_zero(float v) {  // dst[i] = i % 16 != 15 ? src[i + 1] : 0
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x101, 0xF, 0xF, true));
 }
 // value of lane src_lane
 __device__ __forceinline__ float wave_from(float v, int src_lane) {
@@ -105,6 +112,30 @@ __device__ __forceinline__ void shift_y(float (&XA)[N], float (&XB)[N], int d, i
         XA[N - 1] = ta;
         XB[N - 1] = tb;
     }
+}
+
+// In-place slot rotations of a voxel-indexed array (four slots = four voxel rows of the lane):
+// slots_down: v[r] <- v[r - 1], v[0] <- in; slots_up: v[r] <- v[r + 1], v[3] <- in.  Written as
+// register moves on operands tied to their own registers, so that the steps without a y shift
+// keep the array where it is (a phi over a permutation of the slots would otherwise make the
+// compiler copy the array on every step).
+__device__ __forceinline__ void slots_down(float (&v)[4], float in) {
+    asm volatile(
+        "v_mov_b32 %3, %2\n\t"
+        "v_mov_b32 %2, %1\n\t"
+        "v_mov_b32 %1, %0\n\t"
+        "v_mov_b32 %0, %4"
+        : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3])
+        : "v"(in));
+}
+__device__ __forceinline__ void slots_up(float (&v)[4], float in) {
+    asm volatile(
+        "v_mov_b32 %0, %1\n\t"
+        "v_mov_b32 %1, %2\n\t"
+        "v_mov_b32 %2, %3\n\t"
+        "v_mov_b32 %3, %4"
+        : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3])
+        : "v"(in));
 }
 
 // The same for the X state and a third voxel-indexed array that follows the rays (their offsets).
@@ -441,10 +472,11 @@ struct StepRec {
 static_assert(sizeof(StepRec) == 32, "StepRec is two 16-byte LDS accesses");
 // w: [0,5) shift of the midpoint voxel along x (two's complement), [5,10) along y, [10,14) mx & 15,
 //    [14,18) my & 15, bit 18 + [19,23) voxel column whose rays crossed the x edge, bit 23 + [24,28)
-//    voxel row whose rays crossed the y edge, bit 28: a step of more than one cell (test each ray).
+//    voxel row whose rays crossed the y edge, bit 28: a step of more than one cell (test each ray),
+//    bit 29: the previous step's column reset is done by this step's x shift (zero fill).
 constexpr int kRecCap = 32;       // steps per azimuth per chunk
 constexpr int kHdr = kMaxPhi + 2; // per chunk: step count of each azimuth, slice done, next slice
-constexpr uint32_t kColReset = 1u << 18, kRowReset = 1u << 23, kGeneralReset = 1u << 28;
+constexpr uint32_t kColReset = 1u << 18, kRowReset = 1u << 23, kGeneralReset = 1u << 28, kZeroFillX = 1u << 29;
 constexpr int kMaxMarchPhi = 10;  // azimuths of the march kernel (64 KiB of LDS); more: inline kernel
 // A slab in LDS: A/a T, B/b T, -a log2(e) rho, -b log2(e) rho and rho of the slice, 256 each.
 constexpr int kSlab = 5 * kNpix;
@@ -663,10 +695,34 @@ __device__ __forceinline__ void march_chunk(MarchLane& M, const MarchConst& C, c
         // of at most one cell per axis crosses with one ray column and one ray row at most
         if (abs(fnx - fbx) <= 1 && abs(fny - fby) <= 1) {
             const int cxo = crossing_offset(fbx, fnx), cyo = crossing_offset(fby, fny);
-            if (cxo >= 0) w |= kColReset | ((uint32_t)((mx + cxo) & 15) << 19);
+            if (cxo >= 0) {
+                // The column's X is zeroed after this step.  When the next step shifts the state by
+                // one voxel and moves exactly this column across the row end, that shift writes the
+                // zero instead (the state is not read in between), so the reset is left to it.
+                const int col = (mx + cxo) & 15;
+                bool fuse = false;
+                if (j + 1 < c) {
+                    const float4 p2 = P[j + 2];
+                    const int dn = (((int)floorf((p1.x + p2.x) * 0.5f) - mx + 8) & 15) - 8;
+                    fuse = (dn == 1 && col == 15) || (dn == -1 && col == 0);
+                }
+                if (!fuse) w |= kColReset | ((uint32_t)col << 19);
+            }
             if (cyo >= 0) w |= kRowReset | ((uint32_t)((my + cyo) & 15) << 24);
         } else {
             w |= kGeneralReset;
+        }
+        if (j > 0 && (ddx == 1 || ddx == -1)) {
+            // the same decision for the previous step, made by the lane that wrote its record
+            const float4 pm = P[j - 1];
+            const int pbx = (int)floorf(pm.x), pby = (int)floorf(pm.y);
+            if (abs(fbx - pbx) <= 1 && abs(fby - pby) <= 1) {
+                const int pxo = crossing_offset(pbx, fbx);
+                if (pxo >= 0) {
+                    const int pcol = (mxp + pxo) & 15;
+                    if ((ddx == 1 && pcol == 15) || (ddx == -1 && pcol == 0)) w |= kZeroFillX;
+                }
+            }
         }
         const int gxw = (fbx & 15) | ((fnx - fbx) << 4);
         const int gyw = (fby & 15) | ((fny - fby) << 4);
@@ -788,24 +844,54 @@ __device__ __forceinline__ void cccs_compute(const CcBlock& K, const CcLds& S, i
             const int ddy = ((int)(w << 22)) >> 27;
             // rotate the X state (and my rays' x offsets) to follow the midpoint voxel
             if (ddx == 1) {
+                if (w & kZeroFillX) {  // with the previous step's reset of column 15
 #pragma unroll
-                for (int r = 0; r < kSlots; r++) {
-                    XA[r] = row_from_left(XA[r]);
-                    XB[r] = row_from_left(XB[r]);
+                    for (int r = 0; r < kSlots; r++) {
+                        XA[r] = row_from_left_zero(XA[r]);
+                        XB[r] = row_from_left_zero(XB[r]);
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < kSlots; r++) {
+                        XA[r] = row_from_left(XA[r]);
+                        XB[r] = row_from_left(XB[r]);
+                    }
                 }
                 fix = row_from_left(fix);
             } else if (ddx == -1) {
+                if (w & kZeroFillX) {  // with the previous step's reset of column 0
 #pragma unroll
-                for (int r = 0; r < kSlots; r++) {
-                    XA[r] = row_from_right(XA[r]);
-                    XB[r] = row_from_right(XB[r]);
+                    for (int r = 0; r < kSlots; r++) {
+                        XA[r] = row_from_right_zero(XA[r]);
+                        XB[r] = row_from_right_zero(XB[r]);
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < kSlots; r++) {
+                        XA[r] = row_from_right(XA[r]);
+                        XB[r] = row_from_right(XB[r]);
+                    }
                 }
                 fix = row_from_right(fix);
             } else if (ddx != 0) {
                 shift_x(XA, XB, ddx);
                 fix = (float)((vx - (int)((w >> 10) & 15)) & 15);
             }
-            if (ddy != 0) {
+            if (ddy == 1) {
+                const float ta = wave_from(XA[kSlots - 1], up_lane);
+                const float tb = wave_from(XB[kSlots - 1], up_lane);
+                const float tz = wave_from(fiy[kSlots - 1], up_lane);
+                slots_down(XA, ta);
+                slots_down(XB, tb);
+                slots_down(fiy, tz);
+            } else if (ddy == -1) {
+                const float ta = wave_from(XA[0], down_lane);
+                const float tb = wave_from(XB[0], down_lane);
+                const float tz = wave_from(fiy[0], down_lane);
+                slots_up(XA, ta);
+                slots_up(XB, tb);
+                slots_up(fiy, tz);
+            } else if (ddy != 0) {
                 shift_y3(XA, XB, fiy, ddy, up_lane, down_lane);
             }
             const float ddxl = fmaf(ra.x, fix, ra.z);
