@@ -478,15 +478,17 @@ constexpr int kRecCap = 32;       // steps per azimuth per chunk
 constexpr int kHdr = kMaxPhi + 2; // per chunk: step count of each azimuth, slice done, next slice
 constexpr uint32_t kColReset = 1u << 18, kRowReset = 1u << 23, kGeneralReset = 1u << 28, kZeroFillX = 1u << 29;
 constexpr int kMaxMarchPhi = 10;  // azimuths of the march kernel (64 KiB of LDS); more: inline kernel
-// A slab in LDS: A/a T, B/b T, -a log2(e) rho, -b log2(e) rho and rho of the slice, 256 each.
-constexpr int kSlab = 5 * kNpix;
+// A slab in LDS: A/a T, B/b T, -a log2(e) rho and -b log2(e) rho of the slice, 256 each (read by
+// the compute wavefronts when the slice starts), and rho itself (read by the slice's combine).
+constexpr int kSlab = 4 * kNpix;
 constexpr float kLn2 = 0.6931471805599453f;
 
 struct CcLds {
     float4* rec;   // [2][n_phi][kRecStride]: StepRec k of an azimuth at float4 2k, 2k + 1
     float4* pos;   // [n_phi][kRecCap + 1] lattice points of the chunk being marched
     int2* carry;   // [n_phi] (padded to 2 n_phi) midpoint voxel of the last step marched
-    float* slab;   // [3][kSlab], slice i in buffer i % 3 (the combine of slice i - 1 still reads its rho
+    float* slab;   // [2][kSlab], slice i in buffer i & 1
+    float* rho;    // [3][256], slice i in buffer i % 3 (the combine of slice i - 1 still reads its rho
                    // while slice i + 1 is being prefetched)
     float* sC;     // [2][256], slice i in buffer i & 1
     float* sD;     // [2][n_phi][256], slice i in buffer i & 1
@@ -499,7 +501,7 @@ constexpr int kRecStride = 2 * kRecCap + 1;
 
 __host__ __device__ constexpr size_t march_lds_bytes(int n_phi) {
     return (size_t)2 * n_phi * kRecStride * sizeof(float4) + (size_t)n_phi * (kRecCap + 1) * sizeof(float4) +
-           (size_t)2 * n_phi * sizeof(int2) + (size_t)(3 * kSlab + (2 + 2 * n_phi) * kNpix) * sizeof(float) +
+           (size_t)2 * n_phi * sizeof(int2) + (size_t)(2 * kSlab + (3 + 2 + 2 * n_phi) * kNpix) * sizeof(float) +
            (size_t)2 * kHdr * sizeof(int);
 }
 
@@ -512,7 +514,8 @@ __device__ __forceinline__ CcLds carve_lds(float4* base, int n_phi) {
     S.carry = reinterpret_cast<int2*>(S.pos + n_phi * (kRecCap + 1));
     float* f = reinterpret_cast<float*>(S.carry + 2 * n_phi);
     S.slab = f;
-    S.sC = f + 3 * kSlab;
+    S.rho = f + 2 * kSlab;
+    S.sC = S.rho + 3 * kNpix;
     S.sD = S.sC + 2 * kNpix;
     S.hdr = reinterpret_cast<int*>(S.sD + 2 * n_phi * kNpix);
     return S;
@@ -534,15 +537,16 @@ __device__ __forceinline__ float slab_value(const CcBlock& K, int z, int q) {
     const float* src = (q < 4 ? K.Tg : K.Rg) + (uint32_t)z * kNpix + (q & 3) * kWave;
     return src[threadIdx.x & 63];
 }
-// Slab element e (e < 256: Terma, else density; wave-uniform) into the scaled LDS slab.
-__device__ __forceinline__ void store_slab(const CcBlock& K, float* slab, int e, float v) {
+// Slab element e (e < 256: Terma, else density; wave-uniform) of slice ordinal si into LDS.
+__device__ __forceinline__ void store_slab(const CcBlock& K, const CcLds& S, int si, int e, float v) {
+    float* slab = S.slab + (si & 1) * kSlab;
     if (e < kNpix) {
         slab[e] = K.A_a * v;
         slab[kNpix + e] = K.B_b * v;
     } else {
         slab[kNpix + e] = K.na * v;  // 2 kNpix + (e - kNpix)
         slab[2 * kNpix + e] = K.nb * v;
-        slab[3 * kNpix + e] = v;
+        S.rho[(si % 3) * kNpix + e - kNpix] = v;
     }
 }
 __device__ __forceinline__ void slab_regs(const CcBlock& K, int z, float& pf0, float& pf1) {
@@ -555,10 +559,9 @@ __device__ __forceinline__ void slab_regs(const CcBlock& K, int z, float& pf0, f
 __device__ __forceinline__ bool chunk_end(const CcBlock& K, const CcLds& S, int done, int sn, float pf0,
                                           float pf1, int& s, int& si, int& written) {
     if (done && sn < K.L && K.prefetch) {
-        float* nslab = S.slab + ((si + 1) % 3) * kSlab;
         const int i0 = K.tid, i1 = K.tid + K.nthreads;
-        if (i0 < 2 * kNpix) store_slab(K, nslab, i0, pf0);
-        if (K.two && i1 < 2 * kNpix) store_slab(K, nslab, i1, pf1);
+        if (i0 < 2 * kNpix) store_slab(K, S, si + 1, i0, pf0);
+        if (K.two && i1 < 2 * kNpix) store_slab(K, S, si + 1, i1, pf1);
     }
     __syncthreads();
     if (!done) return true;
@@ -573,7 +576,7 @@ __device__ __forceinline__ bool chunk_end(const CcBlock& K, const CcLds& S, int 
             sum = (((sum + v0) + v1) + v2) + v3;
         }
         for (; f < K.n_phi; f++) sum += d[f * kNpix];
-        const float rho = S.slab[(si % 3) * kSlab + 4 * kNpix + i];
+        const float rho = S.rho[(si % 3) * kNpix + i];
         K.P[(uint32_t)s * kNpix + i] = sum * rho * __builtin_amdgcn_rcpf(fmaf(S.sC[(si & 1) * kNpix + i], rho, kEps));
     }
     for (int z = s + 1; z < sn && z < K.L; z++)  // slices the phi-0 march stepped over
@@ -582,9 +585,8 @@ __device__ __forceinline__ bool chunk_end(const CcBlock& K, const CcLds& S, int 
     if (sn >= K.L) return false;
     if (!K.prefetch) {  // small workgroups load the next slab here
         __syncthreads();
-        float* nslab = S.slab + ((si + 1) % 3) * kSlab;
         for (int i = K.tid; i < 2 * kNpix; i += K.nthreads)
-            store_slab(K, nslab, i, i < kNpix ? K.Tg[(uint32_t)sn * kNpix + i] : K.Rg[(uint32_t)sn * kNpix + i - kNpix]);
+            store_slab(K, S, si + 1, i, i < kNpix ? K.Tg[(uint32_t)sn * kNpix + i] : K.Rg[(uint32_t)sn * kNpix + i - kNpix]);
         __syncthreads();
     }
     s = sn;
@@ -823,7 +825,7 @@ __device__ __forceinline__ void cccs_compute(const CcBlock& K, const CcLds& S, i
         float pf0 = 0.0f, pf1 = 0.0f;
         if (done && sn < K.L && K.prefetch) slab_regs(K, sn, pf0, pf1);
         if (fresh) {
-            const float* slab = S.slab + (si % 3) * kSlab + vbase;
+            const float* slab = S.slab + (si & 1) * kSlab + vbase;
 #pragma unroll
             for (int r = 0; r < kSlots; r++) {
                 TA[r] = slab[16 * r];
@@ -984,10 +986,10 @@ __device__ __forceinline__ void cccs_compute(const CcBlock& K, const CcLds& S, i
     zero_tail(K, written);
 }
 
-// 80 VGPRs (6 wavefronts per SIMD) keep two workgroups of 9 wavefronts on a CU; with 96 only one
-// fits.
+// 72 VGPRs (7 wavefronts per SIMD) and 49.6 KiB of LDS fit three workgroups of 9 wavefronts on a CU
+// (the few values the compiler then spills live outside the step loop); 80 VGPRs fit two.
 template <bool EXACT_G>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8))) void cccs_kernel(
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(7, 8))) void cccs_kernel(
     const BeamletDev* __restrict__ beams, int n, const ConeTable* __restrict__ cones, int n_fwd,
     const float* __restrict__ terma, const float* __restrict__ dens_bev, float* __restrict__ partial,
     uint64_t stride) {
@@ -1014,7 +1016,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8))) vo
     K.na = cones->na[t];
     K.nb = cones->nb[t];
     // slab of slice 0 into buffer 0
-    for (int i = K.tid; i < 2 * kNpix; i += K.nthreads) store_slab(K, S.slab, i, i < kNpix ? K.Tg[i] : K.Rg[i - kNpix]);
+    for (int i = K.tid; i < 2 * kNpix; i += K.nthreads) store_slab(K, S, 0, i, i < kNpix ? K.Tg[i] : K.Rg[i - kNpix]);
     if (wave == K.n_phi)
         cccs_march(B, cones, t, K, S);
     else if (wave == K.n_phi - 1)
