@@ -244,7 +244,7 @@ int fd_read_kernel(const char* path, int n_phi, fd_kernel* out) {
         return FD_ERR_ARG;
     }
     if (n_phi <= 0 || n_phi > kMaxPhi) {
-        set_error("n_phi must be in [1, 16]");
+        set_error("n_phi must be in [1, 24]");
         return FD_ERR_ARG;
     }
     std::ifstream f(path);
@@ -318,7 +318,7 @@ int fd_ctx_create(int device, const fd_volume* vol, const float* density_host, c
         return FD_ERR_ARG;
     }
     if (kernel->n_theta <= 0 || kernel->n_theta > FD_MAX_TABLE || kernel->n_phi <= 0 || kernel->n_phi > kMaxPhi) {
-        set_error("kernel must have 1..24 zenith and 1..16 azimuth angles");
+        set_error("kernel must have 1..24 zenith and 1..24 azimuth angles");
         return FD_ERR_ARG;
     }
     for (int d = 0; d < 3; d++)

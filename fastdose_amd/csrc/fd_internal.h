@@ -15,7 +15,7 @@ constexpr int kFmap = 16;
 constexpr int kNpix = kFmap * kFmap;
 constexpr int kWave = 64;
 constexpr int kRaysPerLane = kNpix / kWave;
-constexpr int kMaxPhi = 16;       // one wavefront per azimuth, <= 1024 threads per workgroup
+constexpr int kMaxPhi = 24;       // azimuths per cone (MAX_KERNEL_NUM, kernel.cuh:7)
 constexpr int kSuperSampling = 4; // Terma substeps per slice (TermaComp.cu:65)
 constexpr int kResampleSS = 5;    // BEV -> PVCS supersampling per axis (IMRTgeom.cu:352)
 

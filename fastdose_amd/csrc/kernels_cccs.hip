@@ -179,43 +179,199 @@ __device__ __forceinline__ int crossing_offset(int fb, int fn) {
     return fb == fn ? -1 : (-(fb > fn ? fb : fn)) & 15;
 }
 
-// Body of the CCCS kernel; CUMU: this wavefront carries the last azimuth and accumulates the
-// cumulative p used for the slice normalisation (the other wavefronts skip it).
+// March state of one azimuth in a self-marching wavefront: grid direction, its reciprocals and
+// signs, the march position, the midpoint voxel the X state is aligned to and the integer cells
+// of the position.
+struct SelfMarch {
+    float dx, dy, dz, ix_, iy_, iz_, sgx, sgy, sgz;
+    float bxp, byp, bzp;
+    int mx_prev, my_prev, fbx, fby;
+};
+
+// One slice of one azimuth for a self-marching wavefront: steps until the march reaches the slice
+// end; CUMU: this azimuth is the last one and accumulates the cumulative p of the slice.
 template <bool EXACT_G, bool CUMU>
+__device__ __forceinline__ void self_march_slice(SelfMarch& M, float (&XA)[kSlots], float (&XB)[kSlots],
+                                                 const float (&T)[kSlots], const float (&R)[kSlots],
+                                                 float (&D)[kSlots], float (&Cp)[kSlots], float A_a, float B_b,
+                                                 float na, float nb, float ca, float cb, float bsx, float bsy,
+                                                 float ls, float lim_min, float inv_sad, int vx, int g, int up_lane,
+                                                 int down_lane) {
+    for (int it = 0; it < kMaxStepsPerSlice; it++) {
+        // next lattice point (d_nextPoint, DoseComp.cu:76-111)
+        const float sx = axis_step(M.bxp, M.sgx, M.ix_);
+        const float sy = axis_step(M.byp, M.sgy, M.iy_);
+        const float sz = axis_step(M.bzp, M.sgz, M.iz_);
+        const float st = fminf(fminf(sx, sy), sz);
+        float nx = fmaf(st, M.dx, M.bxp), ny = fmaf(st, M.dy, M.byp), nz = fmaf(st, M.dz, M.bzp);
+        // rint == round here: a tie (.5) is never within 1e-4 of an integer, and the snap is only
+        // applied to values within 1e-4 of one; wave-uniform, taken as a scalar branch
+        const bool flag = __builtin_amdgcn_readfirstlane(fabsf(nz - rintf(nz)) < kEps ? 1 : 0) != 0;
+        if (flag) {
+            nx = fmaf(sz, M.dx, M.bxp);
+            ny = fmaf(sz, M.dy, M.byp);
+            nz = rintf(fmaf(sz, M.dz, M.bzp));
+        }
+
+        // shared midpoint voxel (wave-uniform); rotate the X state to follow it
+        const int mx = __builtin_amdgcn_readfirstlane((int)floorf((M.bxp + nx) * 0.5f));
+        const int my = __builtin_amdgcn_readfirstlane((int)floorf((M.byp + ny) * 0.5f));
+        const int ddx = ((mx - M.mx_prev + 8) & 15) - 8;
+        const int ddy = ((my - M.my_prev + 8) & 15) - 8;
+        M.mx_prev = mx;
+        M.my_prev = my;
+        if (ddx != 0) shift_x(XA, XB, ddx);
+        if (ddy != 0) shift_y(XA, XB, ddy, up_lane, down_lane);
+
+        // segment length terms shared by all rays: physical lateral offset of a ray at lattice
+        // coordinate c is (c - 8) bs f(z), f(z) = (z ls + lim_min) / sad (unwrapped)
+        const float dzs = (nz - M.bzp) * ls;
+        const float fe = fmaf(nz, ls, lim_min) * inv_sad;
+        const float df = dzs * inv_sad;
+        const float gx = bsx * df, gy = bsy * df;
+        const float axc = fmaf(gx, M.bxp - 8.0f, bsx * (nx - M.bxp) * fe);
+        const float ayc = fmaf(gy, M.byp - 8.0f, bsy * (ny - M.byp) * fe);
+        const float dz2 = dzs * dzs;
+        const int fnx = __builtin_amdgcn_readfirstlane((int)floorf(nx));
+        const int fny = __builtin_amdgcn_readfirstlane((int)floorf(ny));
+
+        // ray in my column: lane offset, lateral length term
+        const int ix = (vx - mx) & 15;
+        const float ddxl = fmaf(gx, (float)ix, axc);
+        const float dxz2 = fmaf(ddxl, ddxl, dz2);
+#pragma unroll
+        for (int r = 0; r < kSlots; r++) {
+            const int iy = (4 * g + r - my) & 15;
+            const float ddyl = fmaf(gy, (float)iy, ayc);
+            const float pp = __builtin_amdgcn_sqrtf(fmaf(ddyl, ddyl, dxz2)) * R[r];
+            // e^{-ap}, e^{-bp}
+            const float ea = __builtin_amdgcn_exp2f(na * pp);
+            const float eb = __builtin_amdgcn_exp2f(nb * pp);
+            if (EXACT_G) {
+                const float oma = 1.0f - ea, omb = 1.0f - eb;
+                const float ap = ca * pp, bp = cb * pp;
+                const float ga = ap > 0.0f ? oma * __builtin_amdgcn_rcpf(ap) : 1.0f;
+                const float gb = bp > 0.0f ? omb * __builtin_amdgcn_rcpf(bp) : 1.0f;
+                D[r] = fmaf(fmaf(A_a, fmaf(ga, XA[r] - T[r], T[r]), B_b * fmaf(gb, XB[r] - T[r], T[r])), pp, D[r]);
+            } else {
+                D[r] = fmaf(fmaf(A_a, XA[r], B_b * XB[r]), pp, D[r]);
+            }
+            if (CUMU) Cp[r] += pp;
+            // X <- T (1 - e) + e X = T - e (T - X)
+            XA[r] = fmaf(-ea, T[r] - XA[r], T[r]);
+            XB[r] = fmaf(-eb, T[r] - XB[r], T[r]);
+        }
+
+        // rays that crossed the lateral BEV edge restart with X = 0 (DoseComp.cu:372-377).  A step
+        // that moves one cell crosses the edge with one ray column and one ray row at most.
+        if (abs(fnx - M.fbx) <= 1 && abs(fny - M.fby) <= 1) {
+            const int cxo = crossing_offset(M.fbx, fnx), cyo = crossing_offset(M.fby, fny);
+            if (cxo >= 0) {
+                const bool m = vx == ((mx + cxo) & 15);
+#pragma unroll
+                for (int r = 0; r < kSlots; r++) {
+                    XA[r] = m ? 0.0f : XA[r];
+                    XB[r] = m ? 0.0f : XB[r];
+                }
+            }
+            if (cyo >= 0) {
+                const int yrow = (my + cyo) & 15;  // voxel row of the crossing rays
+                const bool m = g == (yrow >> 2);
+                const int rr = yrow & 3;
+#pragma unroll
+                for (int r = 0; r < kSlots; r++)
+                    if (r == rr) {
+                        XA[r] = m ? 0.0f : XA[r];
+                        XB[r] = m ? 0.0f : XB[r];
+                    }
+            }
+        } else {
+            // general test, ray by ray
+            const bool cx = ((M.fbx + ix) >> 4) != ((fnx + ix) >> 4);
+#pragma unroll
+            for (int r = 0; r < kSlots; r++) {
+                const int iy = (4 * g + r - my) & 15;
+                const bool c = cx || (((M.fby + iy) >> 4) != ((fny + iy) >> 4));
+                XA[r] = c ? 0.0f : XA[r];
+                XB[r] = c ? 0.0f : XB[r];
+            }
+        }
+        M.bxp = nx;
+        M.byp = ny;
+        M.bzp = nz;
+        M.fbx = fnx;
+        M.fby = fny;
+        if (flag) break;
+    }
+}
+
+// Workgroup of self-marching wavefronts (more azimuths than the march kernel's LDS holds):
+// wavefront w carries azimuths APW w .. APW w + APW - 1 (those below n_phi), each with its own X
+// state and march, one after the other within every slice; the wavefront holding the last azimuth
+// (always the last wavefront) accumulates that azimuth's cumulative p.  APW = 2 keeps 17-24 azimuths
+// within 12 wavefronts.
+template <bool EXACT_G, int APW>
 __device__ __forceinline__ void cccs_body(const BeamletDev* __restrict__ beams, int n,
                                           const ConeTable* __restrict__ cones, int n_fwd,
                                           const float* __restrict__ terma, const float* __restrict__ dens_bev,
                                           float* __restrict__ partial, uint64_t stride, float* smem,
                                           int* s_next) {
+    const int tid = threadIdx.x;
+    const int nthreads = blockDim.x;
+    const int n_waves = nthreads >> 6;
+    const int n_phi = cones->n_phi;
     float* sTR = smem;             // [2 buffers][T | rho][256]
     float* sC = smem + 4 * kNpix;  // [2 buffers][256]
-    float* sD = smem + 6 * kNpix;  // [2 buffers][n_phi][256]
+    float* sD = smem + 6 * kNpix;  // [2 buffers][n_waves][256]
 
     const int t = blockIdx.x / n;  // forward-cone slot, widest cone first
     const int b = blockIdx.x - t * n;
-    const int tid = threadIdx.x;
-    const int nthreads = blockDim.x;
-    const int n_phi = nthreads >> 6;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
     const BeamletDev& B = beams[b];
     const int L = B.L;
+    const int az0 = APW * wave;                                   // first azimuth of this wavefront
+    const int n_az = n_phi - az0 < APW ? n_phi - az0 : APW;       // its azimuth count (wave-uniform)
+    const int cumu_slot = (n_phi - 1) - az0;                      // slot of the last azimuth, if here
+    const bool last_wave = wave == n_waves - 1;
 
     // cone constants
     const float A_a = uniform(cones->A_a[t]), B_b = uniform(cones->B_b[t]);
     const float na = cones->na[t], nb = cones->nb[t];  // scalar loads
     const float ca = uniform(cones->a[t]), cb = uniform(cones->b[t]);
-
-    // grid direction of (theta, phi_wave): unit vector scaled to voxel units, renormalised
-    const float ux = cones->ux[t][wave], uy = cones->uy[t][wave], uz = cones->uz[t][wave];
-    const float vxd = ux / B.bs[0], vyd = uy / B.bs[1], vzd = uz / B.ls;
-    const float rn = 1.0f / sqrtf(vxd * vxd + vyd * vyd + vzd * vzd);
-    const float dx = uniform(vxd * rn), dy = uniform(vyd * rn), dz = uniform(vzd * rn);
-    const float ix_ = uniform(1.0f / dx), iy_ = uniform(1.0f / dy), iz_ = uniform(1.0f / dz);
-    const float sgx = dx > 0.0f ? 1.0f : -1.0f, sgy = dy > 0.0f ? 1.0f : -1.0f, sgz = dz > 0.0f ? 1.0f : -1.0f;
-
     const float inv_sad = uniform(1.0f / B.sad);
     const float bsx = uniform(B.bs[0]), bsy = uniform(B.bs[1]), ls = uniform(B.ls), lim_min = uniform(B.lim_min);
+
+    // march state of my azimuths; grid direction of (theta, phi): unit vector scaled to voxel units,
+    // renormalised
+    SelfMarch Mz[APW];
+    float XA[APW][kSlots], XB[APW][kSlots];
+#pragma unroll
+    for (int a = 0; a < APW; a++) {
+        const int f = az0 + a < n_phi ? az0 + a : az0;
+        const float ux = cones->ux[t][f], uy = cones->uy[t][f], uz = cones->uz[t][f];
+        const float vxd = ux / B.bs[0], vyd = uy / B.bs[1], vzd = uz / B.ls;
+        const float rn = 1.0f / sqrtf(vxd * vxd + vyd * vyd + vzd * vzd);
+        SelfMarch& M = Mz[a];
+        M.dx = uniform(vxd * rn);
+        M.dy = uniform(vyd * rn);
+        M.dz = uniform(vzd * rn);
+        M.ix_ = uniform(1.0f / M.dx);
+        M.iy_ = uniform(1.0f / M.dy);
+        M.iz_ = uniform(1.0f / M.dz);
+        M.sgx = M.dx > 0.0f ? 1.0f : -1.0f;
+        M.sgy = M.dy > 0.0f ? 1.0f : -1.0f;
+        M.sgz = M.dz > 0.0f ? 1.0f : -1.0f;
+        M.bxp = 0.5f;
+        M.byp = 0.5f;
+        M.bzp = 0.0f;
+        M.mx_prev = M.my_prev = M.fbx = M.fby = 0;
+#pragma unroll
+        for (int r = 0; r < kSlots; r++) {
+            XA[a][r] = 0.0f;
+            XB[a][r] = 0.0f;
+        }
+    }
 
     // my voxels: x = vx, y = 4 g + r; flat index vbase + 16 r
     const int vx = lane & 15;
@@ -224,22 +380,11 @@ __device__ __forceinline__ void cccs_body(const BeamletDev* __restrict__ beams, 
     const int up_lane = (lane + 48) & 63;    // lane (x, g - 1)
     const int down_lane = (lane + 16) & 63;  // lane (x, g + 1)
 
-    float XA[kSlots], XB[kSlots];
-#pragma unroll
-    for (int r = 0; r < kSlots; r++) {
-        XA[r] = 0.0f;
-        XB[r] = 0.0f;
-    }
-    float bxp = 0.5f, byp = 0.5f, bzp = 0.0f;  // march position of this azimuth
-    int mx_prev = 0, my_prev = 0;              // midpoint voxel the X state is aligned to
-    int fbx = 0, fby = 0;                      // integer cells of the march position
-
     const float* __restrict__ Tg = terma + B.bev_off;
     const float* __restrict__ Rg = dens_bev + B.bev_off;
     float* __restrict__ P = partial + ((uint64_t)b * n_fwd + t) * stride;
-    const bool last_phi = wave == n_phi - 1;
-    const bool two = nthreads < 2 * kNpix;         // each thread prefetches two slab values
-    const bool prefetch = nthreads >= kNpix;        // smaller workgroups reload each slab
+    const bool two = nthreads < 2 * kNpix;    // each thread prefetches two slab values
+    const bool prefetch = nthreads >= kNpix;  // smaller workgroups reload each slab
 
     // slab of slice 0 into buffer 0 (T at [0, 256), rho at [256, 512))
     for (int i = tid; i < 2 * kNpix; i += nthreads) sTR[i] = i < kNpix ? Tg[i] : Rg[i - kNpix];
@@ -268,135 +413,23 @@ __device__ __forceinline__ void cccs_body(const BeamletDev* __restrict__ beams, 
             D[r] = 0.0f;
             Cp[r] = 0.0f;
         }
-
-        for (int it = 0; it < kMaxStepsPerSlice; it++) {
-            // next lattice point (d_nextPoint, DoseComp.cu:76-111)
-            const float sx = axis_step(bxp, sgx, ix_);
-            const float sy = axis_step(byp, sgy, iy_);
-            const float sz = axis_step(bzp, sgz, iz_);
-            const float st = fminf(fminf(sx, sy), sz);
-            float nx = fmaf(st, dx, bxp), ny = fmaf(st, dy, byp), nz = fmaf(st, dz, bzp);
-            // rint == round here: a tie (.5) is never within 1e-4 of an integer, and the snap is
-            // only applied to values within 1e-4 of one
-            // wave-uniform: taken as a scalar branch (one step per slice)
-            const bool flag = __builtin_amdgcn_readfirstlane(fabsf(nz - rintf(nz)) < kEps ? 1 : 0) != 0;
-            if (flag) {
-                nx = fmaf(sz, dx, bxp);
-                ny = fmaf(sz, dy, byp);
-                nz = rintf(fmaf(sz, dz, bzp));
-            }
-
-            // shared midpoint voxel (wave-uniform); rotate the X state to follow it
-            const int mx = __builtin_amdgcn_readfirstlane((int)floorf((bxp + nx) * 0.5f));
-            const int my = __builtin_amdgcn_readfirstlane((int)floorf((byp + ny) * 0.5f));
-            const int ddx = ((mx - mx_prev + 8) & 15) - 8;
-            const int ddy = ((my - my_prev + 8) & 15) - 8;
-            mx_prev = mx;
-            my_prev = my;
-            if (ddx == 1) {
 #pragma unroll
-                for (int r = 0; r < kSlots; r++) {
-                    XA[r] = row_from_left(XA[r]);
-                    XB[r] = row_from_left(XB[r]);
-                }
-            } else if (ddx == -1) {
-#pragma unroll
-                for (int r = 0; r < kSlots; r++) {
-                    XA[r] = row_from_right(XA[r]);
-                    XB[r] = row_from_right(XB[r]);
-                }
-            } else if (ddx != 0) {
-                shift_x(XA, XB, ddx);
-            }
-            if (ddy != 0) shift_y(XA, XB, ddy, up_lane, down_lane);
-
-            // segment length terms shared by all rays: physical lateral offset of a ray at lattice
-            // coordinate c is (c - 8) bs f(z), f(z) = (z ls + lim_min) / sad (unwrapped)
-            const float dzs = (nz - bzp) * ls;
-            const float fe = fmaf(nz, ls, lim_min) * inv_sad;
-            const float df = dzs * inv_sad;
-            const float gx = bsx * df, gy = bsy * df;
-            const float axc = fmaf(gx, bxp - 8.0f, bsx * (nx - bxp) * fe);
-            const float ayc = fmaf(gy, byp - 8.0f, bsy * (ny - byp) * fe);
-            const float dz2 = dzs * dzs;
-            const int fnx = __builtin_amdgcn_readfirstlane((int)floorf(nx));
-            const int fny = __builtin_amdgcn_readfirstlane((int)floorf(ny));
-
-            // ray in my column: lane offset, lateral length term
-            const int ix = (vx - mx) & 15;
-            const float ddxl = fmaf(gx, (float)ix, axc);
-            const float dxz2 = fmaf(ddxl, ddxl, dz2);
-#pragma unroll
-            for (int r = 0; r < kSlots; r++) {
-                const int iy = (4 * g + r - my) & 15;
-                const float ddyl = fmaf(gy, (float)iy, ayc);
-                const float p = __builtin_amdgcn_sqrtf(fmaf(ddyl, ddyl, dxz2)) * R[r];
-                // e^{-ap}, e^{-bp}
-                const float ea = __builtin_amdgcn_exp2f(na * p);
-                const float eb = __builtin_amdgcn_exp2f(nb * p);
-                if (EXACT_G) {
-                    const float oma = 1.0f - ea, omb = 1.0f - eb;
-                    const float ap = ca * p, bp = cb * p;
-                    const float ga = ap > 0.0f ? oma * __builtin_amdgcn_rcpf(ap) : 1.0f;
-                    const float gb = bp > 0.0f ? omb * __builtin_amdgcn_rcpf(bp) : 1.0f;
-                    D[r] = fmaf(fmaf(A_a, fmaf(ga, XA[r] - T[r], T[r]), B_b * fmaf(gb, XB[r] - T[r], T[r])), p,
-                                D[r]);
-                } else {
-                    D[r] = fmaf(fmaf(A_a, XA[r], B_b * XB[r]), p, D[r]);
-                }
-                if (CUMU) Cp[r] += p;
-                // X <- T (1 - e) + e X = T - e (T - X)
-                XA[r] = fmaf(-ea, T[r] - XA[r], T[r]);
-                XB[r] = fmaf(-eb, T[r] - XB[r], T[r]);
-            }
-
-            // rays that crossed the lateral BEV edge restart with X = 0 (DoseComp.cu:372-377).  A step
-            // that moves one cell crosses the edge with one ray column and one ray row at most.
-            if (abs(fnx - fbx) <= 1 && abs(fny - fby) <= 1) {
-                const int cxo = crossing_offset(fbx, fnx), cyo = crossing_offset(fby, fny);
-                if (cxo >= 0) {
-                    const bool m = vx == ((mx + cxo) & 15);
-#pragma unroll
-                    for (int r = 0; r < kSlots; r++) {
-                        XA[r] = m ? 0.0f : XA[r];
-                        XB[r] = m ? 0.0f : XB[r];
-                    }
-                }
-                if (cyo >= 0) {
-                    const int yrow = (my + cyo) & 15;  // voxel row of the crossing rays
-                    const bool m = g == (yrow >> 2);
-                    const int rr = yrow & 3;
-#pragma unroll
-                    for (int r = 0; r < kSlots; r++)
-                        if (r == rr) {
-                            XA[r] = m ? 0.0f : XA[r];
-                            XB[r] = m ? 0.0f : XB[r];
-                        }
-                }
-            } else {
-                // general test, ray by ray
-                const bool cx = ((fbx + ix) >> 4) != ((fnx + ix) >> 4);
-#pragma unroll
-                for (int r = 0; r < kSlots; r++) {
-                    const int iy = (4 * g + r - my) & 15;
-                    const bool c = cx || (((fby + iy) >> 4) != ((fny + iy) >> 4));
-                    XA[r] = c ? 0.0f : XA[r];
-                    XB[r] = c ? 0.0f : XB[r];
-                }
-            }
-            bxp = nx;
-            byp = ny;
-            bzp = nz;
-            fbx = fnx;
-            fby = fny;
-            if (flag) break;
+        for (int a = 0; a < APW; a++) {
+            if (a >= n_az) break;  // wave-uniform
+            if (a == cumu_slot)
+                self_march_slice<EXACT_G, true>(Mz[a], XA[a], XB[a], T, R, D, Cp, A_a, B_b, na, nb, ca, cb, bsx, bsy,
+                                                ls, lim_min, inv_sad, vx, g, up_lane, down_lane);
+            else
+                self_march_slice<EXACT_G, false>(Mz[a], XA[a], XB[a], T, R, D, Cp, A_a, B_b, na, nb, ca, cb, bsx,
+                                                 bsy, ls, lim_min, inv_sad, vx, g, up_lane, down_lane);
         }
 
-        // publish this azimuth's slice dose, the next slab and the next slice index
-        float* myD = sD + (buf * n_phi + wave) * kNpix;
+        // publish this wavefront's slice dose (sum over its azimuths), the next slab and the next
+        // slice index (the phi-0 march decides it)
+        float* myD = sD + (buf * n_waves + wave) * kNpix;
 #pragma unroll
         for (int r = 0; r < kSlots; r++) myD[vbase + 16 * r] = D[r];
-        if (last_phi) {
+        if (last_wave) {
 #pragma unroll
             for (int r = 0; r < kSlots; r++) sC[buf * kNpix + vbase + 16 * r] = Cp[r];
         }
@@ -406,13 +439,13 @@ __device__ __forceinline__ void cccs_body(const BeamletDev* __restrict__ beams, 
             if (i0 < 2 * kNpix) nslab[i0] = pf0;
             if (two && i1 < 2 * kNpix) nslab[i1] = pf1;
         }
-        if (tid == 0) s_next[buf] = max((int)rintf(bzp), s + 1);  // always advances
+        if (tid == 0) s_next[buf] = max((int)rintf(Mz[0].bzp), s + 1);  // always advances
         __syncthreads();
         const int sn = s_next[buf];
         // combine slice s: sum over azimuths / (cumulative p of the last azimuth + 1e-4)
         for (int i = tid; i < kNpix; i += nthreads) {
             float sum = 0.0f;
-            for (int f = 0; f < n_phi; f++) sum += sD[(buf * n_phi + f) * kNpix + i];
+            for (int f = 0; f < n_waves; f++) sum += sD[(buf * n_waves + f) * kNpix + i];
             P[(size_t)s * kNpix + i] = sum / (sC[buf * kNpix + i] + kEps);
         }
         for (int z = s + 1; z < sn && z < L; z++)  // slices the phi-0 march stepped over
@@ -433,18 +466,15 @@ __device__ __forceinline__ void cccs_body(const BeamletDev* __restrict__ beams, 
         for (int i = tid; i < kNpix; i += nthreads) P[(size_t)z * kNpix + i] = 0.0f;
 }
 
-template <bool EXACT_G>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8))) void cccs_inline_kernel(
+// Self-marching workgroups, one or two azimuths per wavefront (APW).
+template <bool EXACT_G, int APW>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 8))) void cccs_inline_kernel(
     const BeamletDev* __restrict__ beams, int n, const ConeTable* __restrict__ cones, int n_fwd,
     const float* __restrict__ terma, const float* __restrict__ dens_bev, float* __restrict__ partial,
     uint64_t stride) {
-    // n_phi = 16: one wavefront per azimuth, each marching itself.  One LDS allocation shared by both instantiations of the body
     extern __shared__ float smem[];
     __shared__ int s_next[2];
-    if ((threadIdx.x >> 6) == (blockDim.x >> 6) - 1)
-        cccs_body<EXACT_G, true>(beams, n, cones, n_fwd, terma, dens_bev, partial, stride, smem, s_next);
-    else
-        cccs_body<EXACT_G, false>(beams, n, cones, n_fwd, terma, dens_bev, partial, stride, smem, s_next);
+    cccs_body<EXACT_G, APW>(beams, n, cones, n_fwd, terma, dens_bev, partial, stride, smem, s_next);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -478,6 +508,7 @@ constexpr int kRecCap = 32;       // steps per azimuth per chunk
 constexpr int kHdr = kMaxPhi + 2; // per chunk: step count of each azimuth, slice done, next slice
 constexpr uint32_t kColReset = 1u << 18, kRowReset = 1u << 23, kGeneralReset = 1u << 28, kZeroFillX = 1u << 29;
 constexpr int kMaxMarchPhi = 10;  // azimuths of the march kernel (64 KiB of LDS); more: inline kernel
+constexpr int kMaxPhiPerWave1 = 16;  // self-marching kernel: one azimuth per wavefront up to 16, then two
 // A slab in LDS: A/a T, B/b T, -a log2(e) rho and -b log2(e) rho of the slice, 256 each (read by
 // the compute wavefronts when the slice starts), and rho itself (read by the slice's combine).
 constexpr int kSlab = 4 * kNpix;
@@ -1082,14 +1113,26 @@ hipError_t launch_cccs(const BeamletDev* beams, int n, int max_L, const ConeTabl
                 hipLaunchKernelGGL(cccs_kernel<false>, grid, block, shm, stream, beams, n, cones_dev, n_fwd, terma,
                                    dens_bev, partial, stride);
         } else {
-            const dim3 block(n_phi * kWave);
-            const size_t shm = (size_t)(6 + 2 * n_phi) * kNpix * sizeof(float);
-            if (exact_g)
-                hipLaunchKernelGGL(cccs_inline_kernel<true>, grid, block, shm, stream, beams, n, cones_dev, n_fwd,
-                                   terma, dens_bev, partial, stride);
-            else
-                hipLaunchKernelGGL(cccs_inline_kernel<false>, grid, block, shm, stream, beams, n, cones_dev, n_fwd,
-                                   terma, dens_bev, partial, stride);
+            // one azimuth per wavefront up to 16 azimuths (1024 threads), two beyond
+            const int apw = n_phi <= kMaxPhiPerWave1 ? 1 : 2;
+            const int waves = (n_phi + apw - 1) / apw;
+            const dim3 block(waves * kWave);
+            const size_t shm = (size_t)(6 + 2 * waves) * kNpix * sizeof(float);
+            if (apw == 1) {
+                if (exact_g)
+                    hipLaunchKernelGGL((cccs_inline_kernel<true, 1>), grid, block, shm, stream, beams, n, cones_dev,
+                                       n_fwd, terma, dens_bev, partial, stride);
+                else
+                    hipLaunchKernelGGL((cccs_inline_kernel<false, 1>), grid, block, shm, stream, beams, n, cones_dev,
+                                       n_fwd, terma, dens_bev, partial, stride);
+            } else {
+                if (exact_g)
+                    hipLaunchKernelGGL((cccs_inline_kernel<true, 2>), grid, block, shm, stream, beams, n, cones_dev,
+                                       n_fwd, terma, dens_bev, partial, stride);
+                else
+                    hipLaunchKernelGGL((cccs_inline_kernel<false, 2>), grid, block, shm, stream, beams, n, cones_dev,
+                                       n_fwd, terma, dens_bev, partial, stride);
+            }
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;

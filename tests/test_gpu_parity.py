@@ -89,10 +89,12 @@ def test_small_exact_g(torch_cuda):
     check_against_oracle(cfg, beams, flu, g, range(len(beams)), flags=orc.EXACT_G)
 
 
-@pytest.mark.parametrize("n_phi", [2, 5, 16])
+@pytest.mark.parametrize("n_phi", [2, 5, 16, 17, 24])
 def test_azimuth_counts(torch_cuda, n_phi):
     """Workgroup shapes other than 8 azimuths: 2 (no slab prefetch), 5 (two slab values per
-    thread) and 16 (the kernel whose wavefronts march themselves)."""
+    thread), 16 (the kernel whose wavefronts march themselves, one azimuth each) and 17 and 24 (two
+    azimuths per self-marching wavefront, the last one alone for 17; the reference accepts up to 24,
+    DoseComp.cu:26-30)."""
     cfg = synth.small_config(seed=11, dim=(22, 24, 22), fluence_dim=2, angles=(40.0, -30.0, 15.0))
     cfg.kernel = synth.synthetic_kernel(1011, n_phi=n_phi)
     beams, flu = beamlets_of(cfg)
