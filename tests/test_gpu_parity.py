@@ -489,3 +489,13 @@ def test_error_paths(torch_cuda):
         with pytest.raises(fd.FastDoseError) as e:
             ctx.cccs([bad], T, T, T, stride)
         assert e.value.code == -6
+        # subFluenceDim 8 (IMRTArgs.cpp --subFluenceDim): the host geometry builds the beamlet, the
+        # whole-path call rejects it before any kernel runs
+        b8 = synth.make_bundle(cfg.bundles[0].isocenter, (30.0, 20.0, 10.0), fluence_dim=1, sub_dim=8, sub_on=2)
+        bb, _ = fd.beamlets_init(b8, np.ones(1, np.uint8), cfg.volume)
+        assert len(bb) == 1 and bb[0].fmap_size[0] == 8
+        dm = fd.DoseMat(16 ** 3)
+        with pytest.raises(fd.FastDoseError) as e:
+            dm.add_beam(ctx, bb, np.zeros((1, 256), np.float32), cfg.extent)
+        assert e.value.code == -6 and dm.info()[0] == 0
+        dm.close()
