@@ -491,7 +491,7 @@ def leg_precision(fd, synth, torch, np, k=256, reps=10):
 
 # ---------------------------------------------------------------------------- serving (config 5)
 
-def leg_serving(fd, synth, workload, torch, np, capacity_beamlets_s, n_req=30):
+def leg_serving(fd, synth, workload, torch, np, capacity_beamlets_s, n_req=60):
     """220 x 220 x 150 body with 1160 candidate beams; Poisson arrivals of 1-20-beam requests at
     several loads and batch sizes.  Per-beam latency = submit -> beam ready (library), request
     latency = submit -> fetched; each request is fetched as soon as it completes, inside the span."""
@@ -527,8 +527,9 @@ def leg_serving(fd, synth, workload, torch, np, capacity_beamlets_s, n_req=30):
     out["plain"] = {"beamlets_per_s": round(nb / plain_s, 1), "beam_latency_ms": {
         "p50": round(percentile(lat, 50), 3), "p99": round(percentile(lat, 99), 3)}}
     first_req = None
-    for batch, loads in ((4096, (0.5, 0.9, 1.5)), (1024, (0.9,)), (2048, (0.9,))):
-        srv = fd.Server(ctx, n_slots=3, max_batch_beamlets=batch, extent=cfg.extent)
+    for batch, slots, loads in ((4096, 3, (0.5, 0.9, 1.5)), (1024, 3, (0.9,)), (2048, 3, (0.9,)), (1024, 2, (0.9,)),
+                                (2048, 2, (0.9,))):
+        srv = fd.Server(ctx, n_slots=slots, max_batch_beamlets=batch, extent=cfg.extent)
         for w in range(3):
             srv.fetch(srv.submit(*payload[w])).close()
         for mult in loads:
@@ -536,7 +537,7 @@ def leg_serving(fd, synth, workload, torch, np, capacity_beamlets_s, n_req=30):
             gaps = rng.exponential(1.0 / rate, size=n_req)
             srv.reset_stats()
             ids, t_sub, t_got, mats = [None] * n_req, [0.0] * n_req, [0.0] * n_req, [None] * n_req
-            ready = threading.Semaphore(0)
+            submitted = [0]
 
             def producer():
                 t_next = time.perf_counter()
@@ -547,30 +548,39 @@ def leg_serving(fd, synth, workload, torch, np, capacity_beamlets_s, n_req=30):
                         time.sleep(d)
                     t_sub[i] = time.perf_counter()
                     ids[i] = srv.submit(*payload[i])
-                    ready.release()
+                    submitted[0] = i + 1
 
             th = threading.Thread(target=producer)
             t_begin = time.perf_counter()
             th.start()
-            for i in range(n_req):       # fetch in submission order as each completes
-                ready.acquire()
-                m = fd.DoseMat(nvox)
-                srv.fetch(ids[i], m)
-                t_got[i] = time.perf_counter()
-                mats[i] = m
+            pending = set()
+            nseen = 0
+            while nseen < n_req or pending:  # fetch each request as soon as it completes
+                while nseen < submitted[0]:
+                    pending.add(nseen)
+                    nseen += 1
+                done = [i for i in sorted(pending) if srv.poll(ids[i]) != 0]
+                for i in done:
+                    m = fd.DoseMat(nvox)
+                    srv.fetch(ids[i], m)
+                    t_got[i] = time.perf_counter()
+                    mats[i] = m
+                    pending.discard(i)
+                if not done:
+                    time.sleep(2e-4)
             th.join()
             span = max(t_got) - min(t_sub)
             st = srv.stats()
             rl = [(b - a) * 1e3 for a, b in zip(t_sub, t_got)]
             out["runs"].append({
-                "max_batch_beamlets": batch, "load": mult, "requests_per_s": round(rate, 2),
+                "max_batch_beamlets": batch, "slots": slots, "load": mult, "requests_per_s": round(rate, 2),
                 "beams": st["beams_done"], "beamlets": st["beamlets_done"],
                 "beamlets_per_s": round(st["beamlets_done"] / span, 1),
                 "beam_latency_ms": {"p50": round(st["p50_ms"], 3), "p99": round(st["p99_ms"], 3)},
                 "request_latency_ms": {"p50": round(percentile(rl, 50), 3), "p99": round(percentile(rl, 99), 3)},
                 "batches": st["batches"], "peak_inflight_batches": st["peak_inflight_batches"],
                 "wall_s": round(time.perf_counter() - t_begin, 3)})
-            log(f"serving batch {batch} load {mult}: {st['beamlets_done'] / span:.0f} beamlets/s, beam p50 "
+            log(f"serving batch {batch} slots {slots} load {mult}: {st['beamlets_done'] / span:.0f} beamlets/s, beam p50 "
                 f"{st['p50_ms']:.1f} ms, p99 {st['p99_ms']:.1f} ms")
             if first_req is None:
                 first_req = (reqs[0], mats[0])
@@ -615,6 +625,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline only (no plain/slab/small/precision/serving)")
     ap.add_argument("--only-slab", action="store_true", help="run only the 128^3 slab leg and print its JSON")
+    ap.add_argument("--only-serving", type=float, default=0.0, metavar="CAPACITY",
+                    help="run only the serving leg, loads relative to CAPACITY beamlets/s, and print its JSON")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 256 MiB device copy first (reference for HBM traffic counters)")
     args = ap.parse_args()
@@ -640,6 +652,9 @@ def main() -> int:
         del src, dst
     if args.only_slab:
         print(json.dumps({"slab": leg_slab(fd, synth, torch, np, args.steps)}), flush=True)
+        return 0
+    if args.only_serving > 0:
+        print(json.dumps({"serving": leg_serving(fd, synth, workload, torch, np, args.only_serving)}), flush=True)
         return 0
 
     head, state = leg_headline(args, fd, synth, workload, torch, np)
@@ -702,7 +717,8 @@ def main() -> int:
         out["small_water"] = leg_small(fd, synth, torch, np, 10)
         out["precision"] = leg_precision(fd, synth, torch, np)
         out["serving"] = leg_serving(fd, synth, workload, torch, np, head["value"])
-        run = next(r for r in out["serving"]["runs"] if r["max_batch_beamlets"] == 4096 and r["load"] == 0.9)
+        run = next(r for r in out["serving"]["runs"]
+                   if r["max_batch_beamlets"] == 4096 and r["slots"] == 3 and r["load"] == 0.9)
         out["latency_ms"] = run["beam_latency_ms"]
     print(json.dumps(out), flush=True)
     return 0

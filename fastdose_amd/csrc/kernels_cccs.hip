@@ -988,19 +988,21 @@ __device__ __forceinline__ void cccs_compute(const CcBlock& K, const CcLds& S, i
                 }
             }
         };
-        // two record register sets in turn, the next record read while a step computes
-        float4 ra = rp[0], qa;
-        float2 rb = rp2[2], qb;
+        // two record register sets in turn, the next record read while a step computes.  The read
+        // one record past the chunk's last (record kRecCap at most: the azimuth's padding float4 and
+        // the first float4 after it, both inside this workgroup's LDS) is never used.
+        const float4* q = rp;
+        float4 ra = q[0], qa;
+        float2 rb = reinterpret_cast<const float2*>(q)[2], qb;
         for (int k = 0; k < cnt; k += 2) {
-            const int k1 = k + 1 < kRecCap ? k + 1 : k;
-            qa = rp[2 * k1];
-            qb = rp2[4 * k1 + 2];
+            qa = q[2];
+            qb = reinterpret_cast<const float2*>(q + 2)[2];
             step(ra, rb, k);
             if (k + 1 >= cnt) break;
-            const int k2 = k + 2 < kRecCap ? k + 2 : k;
-            ra = rp[2 * k2];
-            rb = rp2[4 * k2 + 2];
+            ra = q[4];
+            rb = reinterpret_cast<const float2*>(q + 4)[2];
             step(qa, qb, k + 1);
+            q += 4;
         }
         if (done) {  // publish this azimuth's slice dose (and the last azimuth's cumulative length)
             float* myD = S.sD + ((si & 1) * n_phi + wave) * kNpix;

@@ -109,10 +109,16 @@ void release_beam(fd_server* s, BeamOut& o) {
     o.block = nullptr;
 }
 
-// A block from the pool, or a new one (server lock held).
+// A block from the pool, or a new one (server lock held).  The slot swaps its output buffers with
+// the block's, so the largest free block is taken: the buffers the slot gets back then rarely have
+// to grow for its next batch (growing frees them, and a hipFree waits for the whole device).
 Block* take_block(fd_server* s) {
     if (!s->free_blocks.empty()) {
-        Block* b = s->free_blocks.back();
+        size_t best = 0;
+        for (size_t i = 1; i < s->free_blocks.size(); i++)
+            if (s->free_blocks[i]->cols.cap > s->free_blocks[best]->cols.cap) best = i;
+        Block* b = s->free_blocks[best];
+        s->free_blocks[best] = s->free_blocks.back();
         s->free_blocks.pop_back();
         return b;
     }
@@ -399,12 +405,21 @@ int fd_serve_fetch(fd_server* s, uint64_t id, fd_dosemat* m) {
     }
     // every beam's offsets (re-based to its first row) in one host array that outlives the copies
     std::vector<uint64_t> host_off;
-    if (rc == FD_OK) {
-        size_t total = 0;
-        for (const BeamOut& o : q->out) total += o.rows + 1;
-        host_off.reserve(total);
-    }
     hipStream_t fs = s->fetch_stream;
+    if (rc == FD_OK) {
+        size_t total_off = 0;
+        uint64_t total_nnz = 0;
+        for (const BeamOut& o : q->out) {
+            total_off += o.rows + 1;
+            total_nnz += o.block->rowoff[o.first_row + o.rows] - o.block->rowoff[o.first_row];
+        }
+        host_off.reserve(total_off);
+        // the ensemble grows once for the whole request: growing it beam by beam frees the old
+        // buffers each time, and a hipFree waits for the whole device, slot streams included
+        rc = grow_keep(m->offsets, m->used_off, m->used_off + total_off, fs);
+        if (rc == FD_OK) rc = grow_keep(m->columns, m->used_nnz, m->used_nnz + std::max<uint64_t>(total_nnz, 1), fs);
+        if (rc == FD_OK) rc = grow_keep(m->values, m->used_nnz, m->used_nnz + std::max<uint64_t>(total_nnz, 1), fs);
+    }
     for (size_t k = 0; k < q->out.size() && rc == FD_OK; k++) {
         const BeamOut& o = q->out[k];
         const std::vector<uint64_t>& ro = o.block->rowoff;
