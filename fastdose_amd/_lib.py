@@ -25,6 +25,8 @@ FD_OPT_TIMING = 4
 FD_OPT_RPL_FP64_ALL = 8
 FD_OPT_GRAPH = 16
 FD_OPT_IEEE_DIV = 32
+FD_OPT_TABLES = 64
+FD_OPT_NO_TABLES = 128
 FD_VAL_F32 = 0
 FD_VAL_BF16 = 1
 FD_VAL_FP16 = 2

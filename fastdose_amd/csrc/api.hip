@@ -445,8 +445,12 @@ int fd_ctx_set_options(fd_ctx* ctx, int options) {
         return FD_ERR_ARG;
     }
     if (options & ~(FD_OPT_EXACT_G | FD_OPT_RPL_FP64 | FD_OPT_TIMING | FD_OPT_RPL_FP64_ALL | FD_OPT_GRAPH |
-                    FD_OPT_IEEE_DIV)) {
+                    FD_OPT_IEEE_DIV | FD_OPT_TABLES | FD_OPT_NO_TABLES)) {
         set_error("unknown option bits");
+        return FD_ERR_ARG;
+    }
+    if ((options & FD_OPT_TABLES) && (options & FD_OPT_NO_TABLES)) {
+        set_error("FD_OPT_TABLES and FD_OPT_NO_TABLES exclude each other");
         return FD_ERR_ARG;
     }
     ctx->options = options;

@@ -104,7 +104,16 @@ struct Workspace {
     DevBuf<uint32_t> flags;       // [0]: resampler staging overflow
     DevBuf<uint32_t> rowmax;      // per-beamlet maximum (float bits) for the relative threshold
     DevBuf<uint64_t> scan_tmp;    // tile sums of the multi-workgroup scans
-    uint64_t* host_totals = nullptr;  // pinned: candidates, nonzeros, overflow flag
+    // CCCS march tables of the last batch that used them: entries per (geometry, cone), record
+    // streams, chunk headers, the pinned staging of the entries and each beamlet's geometry
+    DevBuf<TabEntry> tab_ent;
+    DevBuf<float4> tab_rec;
+    DevBuf<int> tab_hdr;
+    TabEntry* tab_stage = nullptr;
+    size_t tab_stage_cap = 0;
+    std::vector<int32_t> tab_geom;
+    int tab_n_gt = 0;
+    uint64_t* host_totals = nullptr;  // pinned: candidates, nonzeros, overflow flags (resampler, tables)
     uint64_t* host_rowoff = nullptr;  // pinned copy of rowoff
     size_t host_rowoff_cap = 0;
     double cand_per_beamlet = 0.0;    // running estimate that sizes the staging up front
@@ -210,7 +219,7 @@ int upload_beams(Workspace& w, const fd_beam* beams, int n, uint64_t stride, hip
 int run_terma(const fd_ctx* ctx, Workspace& w, int options, int n, const float* fluence, float* terma, float* dens,
               uint64_t stride, hipStream_t stream);
 int run_cccs(const fd_ctx* ctx, Workspace& w, int options, int n, int max_L, const float* terma, const float* dens,
-             float* dose, uint64_t stride, hipStream_t stream);
+             float* dose, uint64_t stride, hipStream_t stream, const CccsTables* tab = nullptr);
 // The whole resampler into the caller's CSR arrays `out` (offsets always; columns and values
 // only when nnz <= out.capacity), with one host synchronisation (a second only when the estimated
 // staging overflows).  thr > 0 keeps only entries >= thr * (the beamlet's maximum value).

@@ -29,6 +29,9 @@ void Workspace::release() {
     own_stream = nullptr;
     if (stage) (void)hipHostFree(stage);
     if (flu_stage) (void)hipHostFree(flu_stage);
+    if (tab_stage) (void)hipHostFree(tab_stage);
+    tab_stage = nullptr;
+    tab_stage_cap = 0;
     if (host_totals) (void)hipHostFree(host_totals);
     if (host_rowoff) (void)hipHostFree(host_rowoff);
     host_rowoff = nullptr;
@@ -59,6 +62,9 @@ void Workspace::release() {
     flags.release();
     rowmax.release();
     scan_tmp.release();
+    tab_ent.release();
+    tab_rec.release();
+    tab_hdr.release();
 }
 
 int Workspace::fold(int slot) {
@@ -78,7 +84,8 @@ int Workspace::fold(int slot) {
 size_t Workspace::footprint() const {
     return desc.bytes() + partial.bytes() + fluence.bytes() + terma.bytes() + dens_bev.bytes() + dose_bev.bytes() +
            cand_count.bytes() + nnz_count.bytes() + cand_off.bytes() + nnz_off.bytes() + tmp_cols.bytes() +
-           tmp_vals.bytes() + rowoff.bytes() + out_cols.bytes() + out_vals.bytes() + flags.bytes() + rowmax.bytes();
+           tmp_vals.bytes() + rowoff.bytes() + out_cols.bytes() + out_vals.bytes() + flags.bytes() + rowmax.bytes() +
+           tab_ent.bytes() + tab_rec.bytes() + tab_hdr.bytes();
 }
 
 int check_beams(const fd_beam* beams, int n) {
@@ -242,7 +249,7 @@ int reserve_cccs(const fd_ctx* ctx, Workspace& w, int n, uint64_t stride) {
 }
 
 int run_cccs(const fd_ctx* ctx, Workspace& w, int options, int n, int max_L, const float* terma, const float* dens,
-             float* dose, uint64_t stride, hipStream_t stream) {
+             float* dose, uint64_t stride, hipStream_t stream, const CccsTables* tab) {
     FD_TRY(reserve_cccs(ctx, w, n, stride));
     hipEvent_t evs[3];
     const bool t = w.timing && w.ev[0];
@@ -255,7 +262,7 @@ int run_cccs(const fd_ctx* ctx, Workspace& w, int options, int n, int max_L, con
     }
     hipError_t e = launch_cccs(w.desc.ptr, n, max_L, ctx->cones_dev, ctx->n_fwd, ctx->cones.n_phi,
                                (options & FD_OPT_EXACT_G) != 0, terma, dens, w.partial.ptr, dose, stride, stream,
-                               t ? evs : nullptr);
+                               t ? evs : nullptr, tab);
     if (t) {
         // the cone sum starts where the CCCS kernel stops (timings pair them)
         w.end_mark(FD_T_CCCS);
@@ -276,7 +283,7 @@ int reserve_segments(const fd_ctx* ctx, Workspace& w, int n) {
     FD_TRY(w.nnz_count.reserve(segs));
     FD_TRY(w.cand_off.reserve(segs + 1));
     FD_TRY(w.nnz_off.reserve(segs + 1));
-    FD_TRY(w.flags.reserve(2));  // [0] overflow, [1] zero (read as one u64 with [0])
+    FD_TRY(w.flags.reserve(4));  // [0] overflow, [1] zero (read as one u64 with [0]), [2] table overflow
     FD_TRY(w.rowmax.reserve(n));
     FD_TRY(w.scan_tmp.reserve(scan_scratch_size(segs)));
     return FD_OK;
@@ -400,6 +407,114 @@ int run_resample_compact(const fd_ctx* ctx, Workspace& w, int n, float thr, uint
     return FD_OK;
 }
 
+namespace {
+
+// Beamlets of a batch per march geometry at which the tables replace the march wavefronts by
+// default: below it, walking every geometry's longest beamlet costs about what it saves.
+constexpr int kTabMinShare = 4;
+
+// March tables for a batch (FD_OPT_TABLES, or kTabMinShare beamlets per march geometry unless
+// FD_OPT_NO_TABLES): assigns w.tab_geom, stages one TabEntry per (geometry, forward cone) in
+// w.tab_stage and reserves the streams and headers.  Capacities: within one slice an azimuth
+// crosses at most ceil(|dx / dz|) + 1 x and ceil(|dy / dz|) + 1 y cell boundaries plus the slice
+// end (dx, dy, dz the lattice direction); twice that plus 4 bounds its steps per slice, a chunk
+// takes at most cccs_chunk_steps() of them, and a walk covers at most Lmax slices.  Sets
+// w.tab_n_gt (0: no tables).
+int plan_tables(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams, int n) {
+    w.tab_n_gt = 0;
+    const int n_phi = ctx->cones.n_phi, n_fwd = ctx->n_fwd;
+    if ((options & FD_OPT_TABLES) && n_phi > cccs_table_max_phi()) {
+        set_error("FD_OPT_TABLES: the march tables take at most " + std::to_string(cccs_table_max_phi()) +
+                  " azimuths");
+        return FD_ERR_UNSUPPORTED;
+    }
+    if ((options & FD_OPT_NO_TABLES) || n_fwd <= 0 || n_phi > cccs_table_max_phi()) return FD_OK;
+    struct Geom {
+        uint32_t bits[3];
+        int Lmax;
+    };
+    std::vector<Geom> geoms;
+    w.tab_geom.resize((size_t)n);
+    // open-addressed index of the geometries by their bit patterns
+    size_t slots = 64;
+    while (slots < 2 * (size_t)n) slots *= 2;
+    std::vector<int> slot_geom(slots, -1);
+    for (int i = 0; i < n; i++) {
+        uint32_t k[3];
+        std::memcpy(&k[0], &beams[i].beamlet_size[0], 4);
+        std::memcpy(&k[1], &beams[i].beamlet_size[1], 4);
+        std::memcpy(&k[2], &beams[i].long_spacing, 4);
+        uint64_t hsh = ((uint64_t)k[0] * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)k[1] * 0xC2B2AE3D27D4EB4Full) ^
+                       ((uint64_t)k[2] * 0x165667B19E3779F9ull);
+        size_t h = (size_t)(hsh ^ (hsh >> 29)) & (slots - 1);
+        int g;
+        for (;;) {
+            g = slot_geom[h];
+            if (g < 0 || std::memcmp(geoms[g].bits, k, sizeof(k)) == 0) break;
+            h = (h + 1) & (slots - 1);
+        }
+        if (g < 0) {
+            g = (int)geoms.size();
+            geoms.push_back(Geom{{k[0], k[1], k[2]}, 0});
+            slot_geom[h] = g;
+        }
+        geoms[g].Lmax = std::max(geoms[g].Lmax, (int)beams[i].long_dim);
+        w.tab_geom[i] = g;
+    }
+    const int ng = (int)geoms.size();
+    if (!(options & FD_OPT_TABLES) && n < kTabMinShare * ng) return FD_OK;
+    const size_t n_gt = (size_t)ng * n_fwd;
+    if (n_gt > w.tab_stage_cap) {
+        if (w.tab_stage) FD_HIP(hipHostFree(w.tab_stage));
+        w.tab_stage = nullptr;
+        w.tab_stage_cap = 0;
+        FD_HIP(hipHostMalloc((void**)&w.tab_stage, n_gt * sizeof(TabEntry), hipHostMallocDefault));
+        w.tab_stage_cap = n_gt;
+    }
+    const int chunk = cccs_chunk_steps();
+    uint64_t rec = 0, hdr = 0;
+    for (int g = 0; g < ng; g++) {
+        float b0, b1, ls;
+        std::memcpy(&b0, &geoms[g].bits[0], 4);
+        std::memcpy(&b1, &geoms[g].bits[1], 4);
+        std::memcpy(&ls, &geoms[g].bits[2], 4);
+        for (int t = 0; t < n_fwd; t++) {
+            double steps = 0;
+            for (int f = 0; f < n_phi; f++) {
+                const double az = std::fabs((double)ctx->cones.uz[t][f]) / ls;
+                const double ax = std::fabs((double)ctx->cones.ux[t][f]) / b0;
+                const double ay = std::fabs((double)ctx->cones.uy[t][f]) / b1;
+                steps = std::max(steps, 4.0 + 2.0 * std::ceil(ax / az) + 2.0 * std::ceil(ay / az));
+            }
+            const double L1 = (double)geoms[g].Lmax + 1.0;
+            const double rec_cap = L1 * steps + chunk, chunk_cap = L1 * std::ceil(steps / chunk) + 2.0;
+            if (!(rec_cap < 4e9 && chunk_cap < 4e9)) {
+                set_error("CCCS march table beyond 2^32 records (cone too close to 90 degrees)");
+                return FD_ERR_CAPACITY;
+            }
+            TabEntry& e = w.tab_stage[(size_t)g * n_fwd + t];
+            e.bs0 = b0;
+            e.bs1 = b1;
+            e.ls = ls;
+            e.Lmax = geoms[g].Lmax;
+            e.rec_cap = (uint32_t)rec_cap;
+            e.chunk_cap = (uint32_t)chunk_cap;
+            e.rec_base = rec;
+            e.hdr_base = hdr;
+            e.pad = 0;
+            rec += 2 * (uint64_t)n_phi * e.rec_cap;
+            hdr += (uint64_t)e.chunk_cap * kTabHdr;
+        }
+    }
+    FD_TRY(w.tab_ent.reserve(n_gt));
+    FD_TRY(w.tab_rec.reserve(rec));
+    FD_TRY(w.tab_hdr.reserve(hdr));
+    w.tab_n_gt = (int)n_gt;
+    return FD_OK;
+}
+
+}  // namespace
+
 int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams, int n, const float* fluence,
               float extent, float thr, hipStream_t stream, BatchResult* out) {
     out->rowoff.assign(1, 0);
@@ -427,7 +542,12 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
 
     FD_TRY(reserve_cccs(ctx, w, n, stride));
     FD_TRY(reserve_host_rowoff(w, n));
+    FD_TRY(plan_tables(ctx, w, options, beams, n));
     FD_TRY(stage_inputs(w, beams, n, stride, fluence));
+    const CccsTables tabs{w.tab_ent.ptr, w.tab_rec.ptr, w.tab_hdr.ptr, w.flags.ptr + 2, w.tab_n_gt};
+    if (w.tab_n_gt)
+        for (int i = 0; i < n; i++) w.stage[i].geom = w.tab_geom[i];
+    w.host_totals[3] = 0;
 
     // Everything from the uploads to the host copies of the totals is one stream-ordered sequence
     // with no host synchronisation, so it can run as a captured HIP graph (FD_OPT_GRAPH).
@@ -435,12 +555,18 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
         FD_HIP(hipMemcpyAsync(w.desc.ptr, w.stage, (size_t)n * sizeof(BeamletDev), hipMemcpyHostToDevice, st));
         FD_HIP(hipMemcpyAsync(w.fluence.ptr, w.flu_stage, (size_t)n * kNpix * sizeof(float), hipMemcpyHostToDevice,
                               st));
+        if (w.tab_n_gt)
+            FD_HIP(hipMemcpyAsync(w.tab_ent.ptr, w.tab_stage, (size_t)w.tab_n_gt * sizeof(TabEntry),
+                                  hipMemcpyHostToDevice, st));
         FD_TRY(run_terma(ctx, w, options, n, w.fluence.ptr, w.terma.ptr, w.dens_bev.ptr, stride, st));
-        FD_TRY(run_cccs(ctx, w, options, n, maxL, w.terma.ptr, w.dens_bev.ptr, w.dose_bev.ptr, stride, st));
+        FD_TRY(run_cccs(ctx, w, options, n, maxL, w.terma.ptr, w.dens_bev.ptr, w.dose_bev.ptr, stride, st,
+                        w.tab_n_gt ? &tabs : nullptr));
         FD_TRY(launch_count(ctx, w, n, st));
         FD_TRY(launch_values(ctx, w, n, w.dose_bev.ptr, extent, thr, cap, st));
         FD_TRY(run_resample_compact(ctx, w, n, thr, w.rowoff.ptr, w.out_cols.ptr, w.out_vals.ptr, cap, st));
         FD_TRY(fetch_totals(ctx, w, n, st, true));
+        if (w.tab_n_gt)  // the table overflow flag into the low half of host_totals[3]
+            FD_HIP(hipMemcpyAsync(w.host_totals + 3, w.flags.ptr + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         return FD_OK;
     };
     if (options & FD_OPT_GRAPH) {
@@ -480,6 +606,10 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
     }
     FD_HIP(hipEventRecord(w.stage_done, stream));
     FD_HIP(hipStreamSynchronize(stream));
+    if (w.tab_n_gt && w.host_totals[3]) {
+        set_error("CCCS march table beyond its capacity");
+        return FD_ERR_HIP;
+    }
     const uint64_t ncand = w.host_totals[0];
     const double ratio = est > 0 ? (double)ncand / est : 1.0;
     if (w.host_totals[2]) {

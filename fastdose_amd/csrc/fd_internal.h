@@ -43,7 +43,7 @@ struct BeamletDev {
     float kx, kz;      // sad / bs.x, sad / bs.y
     float inv_ls;      // 1 / long_spacing
     int32_t L;         // long_dim
-    int32_t pad;
+    int32_t geom;      // march geometry of the batch (CCCS march tables), else 0
     uint64_t bev_off;  // float offset of this beamlet's BEV volume
 };
 static_assert(sizeof(BeamletDev) % 8 == 0, "BeamletDev alignment");
@@ -91,6 +91,25 @@ struct ConeTable {
     float uy[FD_MAX_TABLE][kMaxPhi];
     float uz[FD_MAX_TABLE][kMaxPhi];
 };
+
+// CCCS march table of one (march geometry, forward cone): the lattice walk of every azimuth over
+// the longest beamlet of that geometry, chunk by chunk.  The walk and its decisions depend only on
+// the BEV pixel pitch, the longitudinal spacing and the cone, so every beamlet sharing those three
+// values reads the same table.  Records: per azimuth a stream of rec_cap 32-byte records (two
+// float4 each) at float4 index rec_base + 2 rec_cap f; chunk headers: kTabHdr ints per chunk at
+// int index hdr_base, [f] the step count of azimuth f, [kTabHdr - 2] slice done, [kTabHdr - 1]
+// next slice; the chunk after the last holds no steps and ends the walk.
+constexpr int kTabHdr = 16;
+struct TabEntry {
+    float bs0, bs1, ls;  // the geometry (bit patterns shared by its beamlets)
+    int32_t Lmax;        // longest long_dim among them
+    uint32_t rec_cap;    // records per azimuth
+    uint32_t chunk_cap;  // chunk headers, the terminating one included
+    uint64_t rec_base;   // float4 index of azimuth 0's stream
+    uint64_t hdr_base;   // int index of chunk 0's header
+    uint64_t pad;
+};
+static_assert(sizeof(TabEntry) == 48, "TabEntry layout");
 
 void set_error(const std::string& msg);
 

@@ -379,7 +379,7 @@ void fill_beamlet_dev_no_rotation(const fd_beam& b, uint64_t bev_off, BeamletDev
     d.kz = b.sad / b.beamlet_size[1];
     d.inv_ls = 1.0f / b.long_spacing;
     d.L = (int32_t)b.long_dim;
-    d.pad = 0;
+    d.geom = 0;
     d.bev_off = bev_off;
 }
 

@@ -27,12 +27,32 @@ hipError_t launch_terma(const BeamletDev* beams, int n, const float* fluence, co
                         const fd_volume& vol, const SpectrumDev& spec, float rpl64_len, float* terma,
                         float* dens_bev, float* scratch, int segments, hipStream_t stream);
 
+// March tables of a batch (TabEntry): n_gt = (march geometries) x n_fwd entries, the record
+// streams and chunk headers they index, and the flag a table that outgrows its capacity sets.
+struct CccsTables {
+    const TabEntry* ent;
+    float4* rec;
+    int* hdr;
+    uint32_t* overflow;
+    int n_gt;
+};
+
+// The march tables (one wavefront each); clears *overflow first.  n_phi <= cccs_table_max_phi().
+hipError_t launch_cccs_tables(const CccsTables& tab, const ConeTable* cones_dev, int n_fwd, int n_phi,
+                              hipStream_t stream);
+int cccs_table_max_phi();
+int cccs_chunk_steps();  // lattice steps of one azimuth per chunk at most
+
 // CCCS over the forward cones (DoseComp.cu:217-392): per-cone partial BEV doses
 // partial[(b * n_fwd + t) * stride + s * 256 + p], then their sum into dose.
 // ev (nullable): events recorded before the CCCS kernel, between it and the cone sum, and after.
+// tab (nullable, n_phi <= cccs_table_max_phi()): builds the beamlets' march tables first (its
+// entries already on the device, beams[b].geom indexing them) and runs the CCCS from them; the
+// results are those of the march wavefront.  The events then time the tables with the CCCS.
 hipError_t launch_cccs(const BeamletDev* beams, int n, int max_L, const ConeTable* cones_dev, int n_fwd,
                        int n_phi, bool exact_g, const float* terma, const float* dens_bev, float* partial,
-                       float* dose, uint64_t stride, hipStream_t stream, hipEvent_t* ev = nullptr);
+                       float* dose, uint64_t stride, hipStream_t stream, hipEvent_t* ev = nullptr,
+                       const CccsTables* tab = nullptr);
 
 // Resampling into CSR rows.  Stage 1 counts candidate voxels per (beamlet, z-plane).
 hipError_t launch_resample_count(const BeamletDev* beams, int n, const fd_volume& vol, uint32_t* cand_count,

@@ -507,6 +507,7 @@ static_assert(sizeof(StepRec) == 32, "StepRec is two 16-byte LDS accesses");
 constexpr int kRecCap = 32;       // steps per azimuth per chunk
 constexpr int kHdr = kMaxPhi + 2; // per chunk: step count of each azimuth, slice done, next slice
 constexpr uint32_t kColReset = 1u << 18, kRowReset = 1u << 23, kGeneralReset = 1u << 28, kZeroFillX = 1u << 29;
+constexpr uint32_t kShiftBits = 0x3FFu, kResetBits = kColReset | kRowReset | kGeneralReset;
 constexpr int kMaxMarchPhi = 10;  // azimuths of the march kernel (64 KiB of LDS); more: inline kernel
 constexpr int kMaxPhiPerWave1 = 16;  // self-marching kernel: one azimuth per wavefront up to 16, then two
 // A slab in LDS: A/a T, B/b T, -a log2(e) rho and -b log2(e) rho of the slice, 256 each (read by
@@ -537,6 +538,7 @@ __host__ __device__ constexpr size_t march_lds_bytes(int n_phi) {
 }
 
 static_assert(march_lds_bytes(kMaxMarchPhi) <= 65536, "march kernel LDS beyond 64 KiB");
+static_assert(kMaxMarchPhi <= kTabHdr - 2, "a march table chunk header holds every azimuth's count");
 
 __device__ __forceinline__ CcLds carve_lds(float4* base, int n_phi) {
     CcLds S;
@@ -549,6 +551,25 @@ __device__ __forceinline__ CcLds carve_lds(float4* base, int n_phi) {
     S.sC = S.rho + 3 * kNpix;
     S.sD = S.sC + 2 * kNpix;
     S.hdr = reinterpret_cast<int*>(S.sD + 2 * n_phi * kNpix);
+    return S;
+}
+
+// LDS of cccs_tab_kernel: one record area per compute wavefront (each finishes and reads only its
+// own), then the slabs, rho, cumulative length and per-azimuth doses as in carve_lds.  The record
+// areas come first, so the one read past a chunk's last record stays inside the allocation.
+__host__ __device__ constexpr size_t tab_lds_bytes(int n_phi) {
+    return (size_t)n_phi * kRecStride * sizeof(float4) +
+           (size_t)(2 * kSlab + (3 + 2 + 2 * n_phi) * kNpix) * sizeof(float);
+}
+
+__device__ __forceinline__ CcLds carve_lds_tab(float4* base, int n_phi) {
+    CcLds S{};
+    S.rec = base;
+    float* f = reinterpret_cast<float*>(base + n_phi * kRecStride);
+    S.slab = f;
+    S.rho = f + 2 * kSlab;
+    S.sC = S.rho + 3 * kNpix;
+    S.sD = S.sC + 2 * kNpix;
     return S;
 }
 
@@ -647,16 +668,27 @@ struct MarchConst {
     int lane, kper, lf, lk;  // phase 2: lane -> azimuth lf, step lk of each group of kper steps
 };
 
-__device__ __forceinline__ void march_chunk(MarchLane& M, const MarchConst& C, const CcBlock& K, const CcLds& S,
-                                            float4* rec, int* hdr, int s_cur, bool new_slice) {
+// Where phase 2 puts a chunk's records: the march kernel's LDS records (TABLE false), or a march
+// table's per-azimuth streams in global memory (TABLE true; geometry terms only, see TabOut).
+struct TabOut {
+    float4* rec;       // azimuth 0's stream
+    uint32_t cap;      // records per azimuth
+    const int* ofs;    // LDS: records of each azimuth written before this chunk
+};
+
+// Returns false (TABLE only) when a table stream cannot take the chunk; nothing is written then.
+template <bool TABLE>
+__device__ __forceinline__ bool march_chunk(MarchLane& M, const MarchConst& C, int n_phi, const CcLds& S,
+                                            float4* rec, int* hdr, int s_cur, bool new_slice,
+                                            const TabOut& T = TabOut{}) {
     const int lane = C.lane;
     if (new_slice) {
-        M.act = lane < K.n_phi;
+        M.act = lane < n_phi;
         M.nsteps = 0;
     }
     // phase 1: lattice points; pos[f][0] is where the chunk starts
-    float4* mypos = S.pos + (lane < K.n_phi ? lane : 0) * (kRecCap + 1);
-    if (lane < K.n_phi) mypos[0] = make_float4(M.bxp, M.byp, M.bzp, 0.0f);
+    float4* mypos = S.pos + (lane < n_phi ? lane : 0) * (kRecCap + 1);
+    if (lane < n_phi) mypos[0] = make_float4(M.bxp, M.byp, M.bzp, 0.0f);
     int cnt = 0, k = 0;
     for (; k < kRecCap; k++) {
         if (__ballot(M.act) == 0) break;
@@ -683,13 +715,14 @@ __device__ __forceinline__ void march_chunk(MarchLane& M, const MarchConst& C, c
         }
     }
     const bool done = __ballot(M.act) == 0;
-    if (lane < K.n_phi) hdr[lane] = cnt;
+    if (lane < n_phi) hdr[lane] = cnt;
     if (lane == 0) {
         hdr[kMaxPhi] = done ? 1 : 0;
         hdr[kMaxPhi + 1] = max((int)rintf(M.bzp), s_cur + 1);  // the phi-0 march decides the next slice
     }
+    if (TABLE && __ballot(lane < n_phi && (uint32_t)T.ofs[lane] + (uint32_t)cnt > T.cap) != 0) return false;
     // phase 2: records; k is the largest step count of the chunk
-    const bool mine = C.lf < K.n_phi;
+    const bool mine = C.lf < n_phi;
     const int f = mine ? C.lf : 0;
     const int c = hdr[f];
     const float4* P = S.pos + f * (kRecCap + 1);
@@ -759,21 +792,33 @@ __device__ __forceinline__ void march_chunk(MarchLane& M, const MarchConst& C, c
         }
         const int gxw = (fbx & 15) | ((fnx - fbx) << 4);
         const int gyw = (fby & 15) | ((fny - fby) << 4);
-        out[2 * j] = make_float4(gx, gy, axc, ayc);
-        out[2 * j + 1] = make_float4(dzs * dzs, __uint_as_float(w), __int_as_float(gxw), __int_as_float(gyw));
+        if (TABLE) {
+            // the beamlet terms (lim_min, sad) are applied by the compute wavefronts: the lengths
+            // and the x, y offsets of the step as above, before they meet lim_min and 1 / sad
+            const uint32_t idx = (uint32_t)T.ofs[f] + (uint32_t)j;  // < T.cap (checked above)
+            float4* o = T.rec + 2 * ((uint64_t)f * T.cap + idx);
+            o[0] = make_float4(dzs, p1.z, p0.x - 8.0f, p0.y - 8.0f);
+            o[1] = make_float4(C.bsx * (p1.x - p0.x), C.bsy * (p1.y - p0.y), __uint_as_float(w),
+                               __int_as_float((int)(((uint32_t)gxw & 0xFFFFu) | ((uint32_t)gyw << 16))));
+        } else {
+            out[2 * j] = make_float4(gx, gy, axc, ayc);
+            out[2 * j + 1] = make_float4(dzs * dzs, __uint_as_float(w), __int_as_float(gxw), __int_as_float(gyw));
+        }
         if (j == c - 1) S.carry[f] = make_int2(mx, my);  // after every read of this chunk's carry
     }
+    return true;
 }
 
-__device__ __forceinline__ void cccs_march(const BeamletDev& B, const ConeTable* __restrict__ cones, int t,
-                                           const CcBlock& K, const CcLds& S) {
+// March state of this lane (azimuth lane, if below n_phi) at the start of the walk of cone t, for a
+// beamlet with BEV pitch (bs0, bs1) and longitudinal spacing ls; phase-2 lane assignment.
+__device__ __forceinline__ void march_init(MarchLane& M, MarchConst& C, const ConeTable* __restrict__ cones, int t,
+                                           int n_phi, float bs0, float bs1, float ls) {
     const int lane = threadIdx.x & 63;
-    const int f = lane < K.n_phi ? lane : 0;
-    MarchLane M;
+    const int f = lane < n_phi ? lane : 0;
     {
         // grid direction of (theta, phi_f): unit vector scaled to voxel units, renormalised
         const float ux = cones->ux[t][f], uy = cones->uy[t][f], uz = cones->uz[t][f];
-        const float vxd = ux / B.bs[0], vyd = uy / B.bs[1], vzd = uz / B.ls;
+        const float vxd = ux / bs0, vyd = uy / bs1, vzd = uz / ls;
         const float rn = 1.0f / sqrtf(vxd * vxd + vyd * vyd + vzd * vzd);
         M.dx = vxd * rn;
         M.dy = vyd * rn;
@@ -790,20 +835,27 @@ __device__ __forceinline__ void cccs_march(const BeamletDev& B, const ConeTable*
     M.bzp = 0.0f;
     M.nsteps = 0;
     M.act = false;
-    MarchConst C;
-    C.inv_sad = uniform(1.0f / B.sad);
-    C.bsx = uniform(B.bs[0]);
-    C.bsy = uniform(B.bs[1]);
-    C.ls = uniform(B.ls);
-    C.lim_min = uniform(B.lim_min);
+    C.bsx = uniform(bs0);
+    C.bsy = uniform(bs1);
+    C.ls = uniform(ls);
     C.lane = lane;
-    C.kper = 64 / K.n_phi;
-    C.lk = lane / K.n_phi;
-    C.lf = C.lk < C.kper ? lane - C.lk * K.n_phi : K.n_phi;  // n_phi: idle in phase 2
+    C.kper = 64 / n_phi;
+    C.lk = lane / n_phi;
+    C.lf = C.lk < C.kper ? lane - C.lk * n_phi : n_phi;  // n_phi: idle in phase 2
+}
+
+__device__ __forceinline__ void cccs_march(const BeamletDev& B, const ConeTable* __restrict__ cones, int t,
+                                           const CcBlock& K, const CcLds& S) {
+    const int lane = threadIdx.x & 63;
+    MarchLane M;
+    MarchConst C;
+    march_init(M, C, cones, t, K.n_phi, B.bs[0], B.bs[1], B.ls);
+    C.inv_sad = uniform(1.0f / B.sad);
+    C.lim_min = uniform(B.lim_min);
     if (lane < K.n_phi) S.carry[lane] = make_int2(0, 0);  // the midpoint voxel starts at (0, 0)
     const int n_phi = K.n_phi;
 
-    march_chunk(M, C, K, S, S.rec, S.hdr, 0, true);
+    march_chunk<false>(M, C, K.n_phi, S, S.rec, S.hdr, 0, true);
     __syncthreads();
     int s = 0, buf = 0, si = 0, written = 0;  // si: ordinal of the slice being processed
     for (;;) {
@@ -811,19 +863,98 @@ __device__ __forceinline__ void cccs_march(const BeamletDev& B, const ConeTable*
         float pf0 = 0.0f, pf1 = 0.0f;
         if (done && sn < K.L && K.prefetch) slab_regs(K, sn, pf0, pf1);
         if (!(done && sn >= K.L))
-            march_chunk(M, C, K, S, S.rec + (buf ^ 1) * n_phi * kRecStride, S.hdr + (buf ^ 1) * kHdr,
-                        done ? sn : s, done != 0);
+            march_chunk<false>(M, C, n_phi, S, S.rec + (buf ^ 1) * n_phi * kRecStride, S.hdr + (buf ^ 1) * kHdr,
+                               done ? sn : s, done != 0);
         if (!chunk_end(K, S, done, sn, pf0, pf1, s, si, written)) break;
         buf ^= 1;
     }
     zero_tail(K, written);
 }
 
+// March tables: one wavefront per (march geometry, forward cone) walks the lattice of every azimuth
+// over the geometry's longest beamlet, chunk by chunk exactly as the march wavefront of
+// cccs_kernel does, and writes the chunk headers and geometry records that the compute wavefronts
+// of every beamlet of that geometry read (cccs_tab_kernel).  A stream or header array too small
+// for the walk sets *overflow and ends the table early; the caller reports it.
+__host__ __device__ constexpr size_t table_lds_bytes(int n_phi) {
+    return (size_t)n_phi * (kRecCap + 1) * sizeof(float4) + (size_t)n_phi * sizeof(int2) +
+           (size_t)(kHdr + kWave) * sizeof(int);
+}
+
+__global__ __launch_bounds__(64) void cccs_table_kernel(const TabEntry* __restrict__ tab,
+                                                        const ConeTable* __restrict__ cones, int n_fwd, int n_phi,
+                                                        float4* __restrict__ rec, int* __restrict__ hdr_g,
+                                                        uint32_t* __restrict__ overflow) {
+    extern __shared__ float4 lds4[];
+    const int t = blockIdx.x % n_fwd;
+    const TabEntry E = tab[blockIdx.x];
+    CcLds S{};
+    S.pos = lds4;
+    S.carry = reinterpret_cast<int2*>(S.pos + n_phi * (kRecCap + 1));
+    int* hdr = reinterpret_cast<int*>(S.carry + n_phi);
+    int* ofs = hdr + kHdr;  // records of each azimuth written so far
+    const int lane = threadIdx.x;
+    MarchLane M;
+    MarchConst C;
+    march_init(M, C, cones, t, n_phi, E.bs0, E.bs1, E.ls);
+    if (lane < n_phi) S.carry[lane] = make_int2(0, 0);
+    ofs[lane] = 0;
+    int* H = hdr_g + E.hdr_base;
+    const TabOut T{rec + E.rec_base, E.rec_cap, ofs};
+    uint32_t c = 0;
+    int s = 0;
+    bool first = true, ok = true;
+    for (;;) {
+        // the chunk sequence of cccs_march: the next slice is decided by the phi-0 march
+        int s_cur = 0;
+        bool new_slice = true;
+        if (!first) {
+            const int done = hdr[kMaxPhi], sn = hdr[kMaxPhi + 1];
+            if (done && sn >= E.Lmax) break;
+            if (done) s = sn;
+            s_cur = s;
+            new_slice = done != 0;
+        }
+        first = false;
+        if (c + 1 >= E.chunk_cap || !march_chunk<true>(M, C, n_phi, S, nullptr, hdr, s_cur, new_slice, T)) {
+            ok = false;
+            break;
+        }
+        int* h = H + (uint64_t)c * kTabHdr;
+        if (lane < n_phi) {
+            h[lane] = hdr[lane];
+            ofs[lane] += hdr[lane];
+        }
+        if (lane == 0) {
+            h[kTabHdr - 2] = hdr[kMaxPhi];
+            h[kTabHdr - 1] = hdr[kMaxPhi + 1];
+        }
+        c++;
+    }
+    // the terminating chunk: no steps, slice done, next slice beyond every beamlet
+    int* h = H + (uint64_t)(c < E.chunk_cap ? c : E.chunk_cap - 1) * kTabHdr;
+    if (lane < kTabHdr) h[lane] = lane == kTabHdr - 2 ? 1 : (lane == kTabHdr - 1 ? (1 << 30) : 0);
+    if (!ok && lane == 0) overflow[0] = 1u;
+}
+
+// What a compute wavefront of cccs_tab_kernel reads its chunks from: the chunk headers of its
+// (geometry, cone) table, its azimuth's record stream, and the beamlet terms the geometry records
+// are finished with.
+struct TabRead {
+    const int* hdr;      // chunk 0's header
+    const float4* rec;   // this azimuth's stream
+    uint32_t cap;        // records in it
+    uint32_t chunks;     // chunk headers (the walk ends at the terminating one before them)
+    float lim_min, inv_sad, bsx, bsy, ls;
+};
+
 // Compute wavefront of azimuth `wave`: lane l owns voxels (x = l & 15, y = 4 (l >> 4) + r).  The
 // X states are kept pre-scaled by A/a and B/b, and the dose and cumulative length are summed in
-// units of the geometric length s (p = s rho), multiplied by rho in the combine.
-template <bool EXACT_G, bool CUMU>
-__device__ __forceinline__ void cccs_compute(const CcBlock& K, const CcLds& S, int wave) {
+// units of the geometric length s (p = s rho), multiplied by rho in the combine.  TAB: the chunks
+// come from a march table (cccs_tab_kernel), else from the march wavefront's LDS records.
+template <bool EXACT_G, bool CUMU, bool TAB = false>
+__device__ __forceinline__ void cccs_compute(const CcBlock& K, const CcLds& S, int wave,
+                                             const TabRead& TR = TabRead{}) {
     const int lane = threadIdx.x & 63;
     const int n_phi = K.n_phi;
 
@@ -849,10 +980,42 @@ __device__ __forceinline__ void cccs_compute(const CcBlock& K, const CcLds& S, i
 
     __syncthreads();  // chunk 0 and slab 0 are in LDS
     int s = 0, buf = 0, si = 0, written = 0;  // si: ordinal of the slice being processed
+    uint32_t tc = 0, tofs = 0;                 // TAB: chunk, records of my azimuth before it
     bool fresh = true;
     float TA[kSlots], TB[kSlots], nRa[kSlots], nRb[kSlots], D[kSlots], Cs[kSlots];
     for (;;) {
-        const int done = S.hdr[buf * kHdr + kMaxPhi], sn = S.hdr[buf * kHdr + kMaxPhi + 1];
+        int done, sn, cnt;
+        const float4* rp;
+        if (TAB) {
+            // this chunk's header, and my azimuth's records finished with the beamlet terms into my
+            // LDS records (the arithmetic of the march wavefront's phase 2, same operation order)
+            const int* h = TR.hdr + (uint64_t)min(tc, TR.chunks - 1) * kTabHdr;
+            done = h[kTabHdr - 2];
+            sn = h[kTabHdr - 1];
+            cnt = min(max(h[wave], 0), kRecCap);
+            float4* my = S.rec + wave * kRecStride;
+            // every lane loads (a clamped index keeps the spare lanes inside the stream)
+            const uint32_t idx = min(tofs + (uint32_t)lane, TR.cap - 1);
+            const float4 g0 = TR.rec[2 * (uint64_t)idx], g1 = TR.rec[2 * (uint64_t)idx + 1];
+            // g0 = (dzs, z1, x0 - 8, y0 - 8), g1 = (bsx dx, bsy dy, w, gx | gy << 16)
+            const float fe = fmaf(g0.y, TR.ls, TR.lim_min) * TR.inv_sad;
+            const float df = g0.x * TR.inv_sad;
+            const float gx = TR.bsx * df, gy = TR.bsy * df;
+            const int gxy = __float_as_int(g1.w);
+            if (lane < cnt) {
+                my[2 * lane] = make_float4(gx, gy, fmaf(gx, g0.z, g1.x * fe), fmaf(gy, g0.w, g1.y * fe));
+                my[2 * lane + 1] = make_float4(g0.x * g0.x, g1.z, __int_as_float((int)(short)(gxy & 0xFFFF)),
+                                               __int_as_float(gxy >> 16));
+            }
+            rp = my;
+            tofs += (uint32_t)cnt;
+            tc++;
+        } else {
+            done = S.hdr[buf * kHdr + kMaxPhi];
+            sn = S.hdr[buf * kHdr + kMaxPhi + 1];
+            cnt = __builtin_amdgcn_readfirstlane(S.hdr[buf * kHdr + wave]);
+            rp = S.rec + (buf * n_phi + wave) * kRecStride;
+        }
         float pf0 = 0.0f, pf1 = 0.0f;
         if (done && sn < K.L && K.prefetch) slab_regs(K, sn, pf0, pf1);
         if (fresh) {
@@ -867,65 +1030,67 @@ __device__ __forceinline__ void cccs_compute(const CcBlock& K, const CcLds& S, i
                 Cs[r] = 0.0f;
             }
         }
-        const int cnt = __builtin_amdgcn_readfirstlane(S.hdr[buf * kHdr + wave]);
-        const float4* rp = S.rec + (buf * n_phi + wave) * kRecStride;
         const float2* rp2 = reinterpret_cast<const float2*>(rp);  // (dz2, w) of record k at 4 k + 2
         // one step from its record (ra, rb)
         auto step = [&](const float4& ra, const float2& rb, int k) {
             const uint32_t w = __builtin_amdgcn_readfirstlane(__float_as_uint(rb.y));
             const int ddx = ((int)(w << 27)) >> 27;
             const int ddy = ((int)(w << 22)) >> 27;
-            // rotate the X state (and my rays' x offsets) to follow the midpoint voxel
-            if (ddx == 1) {
-                if (w & kZeroFillX) {  // with the previous step's reset of column 15
+            // steps with no shift of the midpoint voxel (and with no reset below) skip the decision
+            // trees with one scalar test each
+            if (w & kShiftBits) {
+                // rotate the X state (and my rays' x offsets) to follow the midpoint voxel
+                if (ddx == 1) {
+                    if (w & kZeroFillX) {  // with the previous step's reset of column 15
 #pragma unroll
-                    for (int r = 0; r < kSlots; r++) {
-                        XA[r] = row_from_left_zero(XA[r]);
-                        XB[r] = row_from_left_zero(XB[r]);
-                    }
-                } else {
+                        for (int r = 0; r < kSlots; r++) {
+                            XA[r] = row_from_left_zero(XA[r]);
+                            XB[r] = row_from_left_zero(XB[r]);
+                        }
+                    } else {
 #pragma unroll
-                    for (int r = 0; r < kSlots; r++) {
-                        XA[r] = row_from_left(XA[r]);
-                        XB[r] = row_from_left(XB[r]);
+                        for (int r = 0; r < kSlots; r++) {
+                            XA[r] = row_from_left(XA[r]);
+                            XB[r] = row_from_left(XB[r]);
+                        }
                     }
+                    fix = row_from_left(fix);
+                } else if (ddx == -1) {
+                    if (w & kZeroFillX) {  // with the previous step's reset of column 0
+#pragma unroll
+                        for (int r = 0; r < kSlots; r++) {
+                            XA[r] = row_from_right_zero(XA[r]);
+                            XB[r] = row_from_right_zero(XB[r]);
+                        }
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < kSlots; r++) {
+                            XA[r] = row_from_right(XA[r]);
+                            XB[r] = row_from_right(XB[r]);
+                        }
+                    }
+                    fix = row_from_right(fix);
+                } else if (ddx != 0) {
+                    shift_x(XA, XB, ddx);
+                    fix = (float)((vx - (int)((w >> 10) & 15)) & 15);
                 }
-                fix = row_from_left(fix);
-            } else if (ddx == -1) {
-                if (w & kZeroFillX) {  // with the previous step's reset of column 0
-#pragma unroll
-                    for (int r = 0; r < kSlots; r++) {
-                        XA[r] = row_from_right_zero(XA[r]);
-                        XB[r] = row_from_right_zero(XB[r]);
-                    }
-                } else {
-#pragma unroll
-                    for (int r = 0; r < kSlots; r++) {
-                        XA[r] = row_from_right(XA[r]);
-                        XB[r] = row_from_right(XB[r]);
-                    }
+                if (ddy == 1) {
+                    const float ta = wave_from(XA[kSlots - 1], up_lane);
+                    const float tb = wave_from(XB[kSlots - 1], up_lane);
+                    const float tz = wave_from(fiy[kSlots - 1], up_lane);
+                    slots_down(XA, ta);
+                    slots_down(XB, tb);
+                    slots_down(fiy, tz);
+                } else if (ddy == -1) {
+                    const float ta = wave_from(XA[0], down_lane);
+                    const float tb = wave_from(XB[0], down_lane);
+                    const float tz = wave_from(fiy[0], down_lane);
+                    slots_up(XA, ta);
+                    slots_up(XB, tb);
+                    slots_up(fiy, tz);
+                } else if (ddy != 0) {
+                    shift_y3(XA, XB, fiy, ddy, up_lane, down_lane);
                 }
-                fix = row_from_right(fix);
-            } else if (ddx != 0) {
-                shift_x(XA, XB, ddx);
-                fix = (float)((vx - (int)((w >> 10) & 15)) & 15);
-            }
-            if (ddy == 1) {
-                const float ta = wave_from(XA[kSlots - 1], up_lane);
-                const float tb = wave_from(XB[kSlots - 1], up_lane);
-                const float tz = wave_from(fiy[kSlots - 1], up_lane);
-                slots_down(XA, ta);
-                slots_down(XB, tb);
-                slots_down(fiy, tz);
-            } else if (ddy == -1) {
-                const float ta = wave_from(XA[0], down_lane);
-                const float tb = wave_from(XB[0], down_lane);
-                const float tz = wave_from(fiy[0], down_lane);
-                slots_up(XA, ta);
-                slots_up(XB, tb);
-                slots_up(fiy, tz);
-            } else if (ddy != 0) {
-                shift_y3(XA, XB, fiy, ddy, up_lane, down_lane);
             }
             const float ddxl = fmaf(ra.x, fix, ra.z);
             const float dxz2 = fmaf(ddxl, ddxl, rb.x);
@@ -951,40 +1116,42 @@ __device__ __forceinline__ void cccs_compute(const CcBlock& K, const CcLds& S, i
                 XB[r] = fmaf(-eb, TB[r] - XB[r], TB[r]);
             }
             // rays that crossed the lateral BEV edge restart with X = 0 (DoseComp.cu:372-377)
-            if (w & kColReset) {
-                const bool m = vx == (int)((w >> 19) & 15);
+            if (w & kResetBits) {
+                if (w & kColReset) {
+                    const bool m = vx == (int)((w >> 19) & 15);
 #pragma unroll
-                for (int r = 0; r < kSlots; r++) {
-                    XA[r] = m ? 0.0f : XA[r];
-                    XB[r] = m ? 0.0f : XB[r];
-                }
-            }
-            if (w & kRowReset) {
-                const int yrow = (int)((w >> 24) & 15);
-                const bool m = g == (yrow >> 2);
-                const int rr = yrow & 3;
-#pragma unroll
-                for (int r = 0; r < kSlots; r++)
-                    if (r == rr) {
+                    for (int r = 0; r < kSlots; r++) {
                         XA[r] = m ? 0.0f : XA[r];
                         XB[r] = m ? 0.0f : XB[r];
                     }
-            }
-            if (w & kGeneralReset) {
-                const int mx = (int)((w >> 10) & 15), my = (int)((w >> 14) & 15);
-                const float2 gw = rp2[4 * k + 3];
-                const int gxw = __builtin_amdgcn_readfirstlane(__float_as_int(gw.x));
-                const int gyw = __builtin_amdgcn_readfirstlane(__float_as_int(gw.y));
-                const int fbx = gxw & 15, fnx = fbx + (gxw >> 4);
-                const int fby = gyw & 15, fny = fby + (gyw >> 4);
-                const int ix = (vx - mx) & 15;
-                const bool cx = ((fbx + ix) >> 4) != ((fnx + ix) >> 4);
+                }
+                if (w & kRowReset) {
+                    const int yrow = (int)((w >> 24) & 15);
+                    const bool m = g == (yrow >> 2);
+                    const int rr = yrow & 3;
 #pragma unroll
-                for (int r = 0; r < kSlots; r++) {
-                    const int iy = (4 * g + r - my) & 15;
-                    const bool c = cx || (((fby + iy) >> 4) != ((fny + iy) >> 4));
-                    XA[r] = c ? 0.0f : XA[r];
-                    XB[r] = c ? 0.0f : XB[r];
+                    for (int r = 0; r < kSlots; r++)
+                        if (r == rr) {
+                            XA[r] = m ? 0.0f : XA[r];
+                            XB[r] = m ? 0.0f : XB[r];
+                        }
+                }
+                if (w & kGeneralReset) {
+                    const int mx = (int)((w >> 10) & 15), my = (int)((w >> 14) & 15);
+                    const float2 gw = rp2[4 * k + 3];
+                    const int gxw = __builtin_amdgcn_readfirstlane(__float_as_int(gw.x));
+                    const int gyw = __builtin_amdgcn_readfirstlane(__float_as_int(gw.y));
+                    const int fbx = gxw & 15, fnx = fbx + (gxw >> 4);
+                    const int fby = gyw & 15, fny = fby + (gyw >> 4);
+                    const int ix = (vx - mx) & 15;
+                    const bool cx = ((fbx + ix) >> 4) != ((fnx + ix) >> 4);
+#pragma unroll
+                    for (int r = 0; r < kSlots; r++) {
+                        const int iy = (4 * g + r - my) & 15;
+                        const bool c = cx || (((fby + iy) >> 4) != ((fny + iy) >> 4));
+                        XA[r] = c ? 0.0f : XA[r];
+                        XB[r] = c ? 0.0f : XB[r];
+                    }
                 }
             }
         };
@@ -1058,6 +1225,55 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(7, 8))) vo
         cccs_compute<EXACT_G, false>(K, S, wave);
 }
 
+// cccs_kernel without the march wavefront: the workgroup's n_phi compute wavefronts read their
+// chunks from the march table of the beamlet's geometry (cccs_table_kernel) and finish the
+// records with the beamlet's lim_min and sad themselves.  Same arithmetic, same results.
+template <bool EXACT_G>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8))) void cccs_tab_kernel(
+    const BeamletDev* __restrict__ beams, int n, const ConeTable* __restrict__ cones, int n_fwd,
+    const float* __restrict__ terma, const float* __restrict__ dens_bev, float* __restrict__ partial,
+    uint64_t stride, const TabEntry* __restrict__ tab, const float4* __restrict__ tab_rec,
+    const int* __restrict__ tab_hdr) {
+    extern __shared__ float4 lds4[];
+    const int t = blockIdx.x / n;  // forward-cone slot, widest cone first
+    const int b = blockIdx.x - t * n;
+    const BeamletDev& B = beams[b];
+    CcBlock K;
+    K.L = B.L;
+    if (K.L <= 0) return;
+    K.tid = threadIdx.x;
+    K.nthreads = blockDim.x;
+    K.n_phi = blockDim.x >> 6;
+    K.Tg = terma + B.bev_off;
+    K.Rg = dens_bev + B.bev_off;
+    K.P = partial + ((uint64_t)b * n_fwd + t) * stride;
+    K.two = K.nthreads < 2 * kNpix;
+    K.prefetch = K.nthreads >= kNpix;
+    const CcLds S = carve_lds_tab(lds4, K.n_phi);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+
+    K.A_a = cones->A_a[t];
+    K.B_b = cones->B_b[t];
+    K.na = cones->na[t];
+    K.nb = cones->nb[t];
+    const TabEntry& E = tab[(uint32_t)B.geom * (uint32_t)n_fwd + (uint32_t)t];
+    TabRead TR;
+    TR.hdr = tab_hdr + E.hdr_base;
+    TR.rec = tab_rec + E.rec_base + 2 * (uint64_t)wave * E.rec_cap;
+    TR.cap = E.rec_cap;
+    TR.chunks = E.chunk_cap;
+    TR.lim_min = B.lim_min;
+    TR.inv_sad = 1.0f / B.sad;
+    TR.bsx = B.bs[0];
+    TR.bsy = B.bs[1];
+    TR.ls = B.ls;
+    for (int i = K.tid; i < 2 * kNpix; i += K.nthreads) store_slab(K, S, 0, i, i < kNpix ? K.Tg[i] : K.Rg[i - kNpix]);
+    if (wave == K.n_phi - 1)
+        cccs_compute<EXACT_G, true, true>(K, S, wave, TR);
+    else
+        cccs_compute<EXACT_G, false, true>(K, S, wave, TR);
+}
+
 // dose[b][s][p] = sum over forward cones of partial[b][t][s][p], cone slots in order.
 __global__ __launch_bounds__(256) void cone_sum_kernel(const BeamletDev* __restrict__ beams, int n_fwd,
                                                        const float* __restrict__ partial,
@@ -1097,12 +1313,39 @@ __global__ __launch_bounds__(256) void cone_sum4_kernel(const BeamletDev* __rest
 
 }  // namespace
 
+int cccs_table_max_phi() { return kMaxMarchPhi; }
+int cccs_chunk_steps() { return kRecCap; }
+
+hipError_t launch_cccs_tables(const CccsTables& tab, const ConeTable* cones_dev, int n_fwd, int n_phi,
+                              hipStream_t stream) {
+    if (n_phi < 1 || n_phi > kMaxMarchPhi) return hipErrorInvalidValue;
+    hipError_t e = launch_fill_u32(tab.overflow, 1, 0u, stream);  // a kernel, also under graph capture
+    if (e != hipSuccess || tab.n_gt <= 0) return e;
+    hipLaunchKernelGGL(cccs_table_kernel, dim3((unsigned)tab.n_gt), dim3(kWave), table_lds_bytes(n_phi), stream,
+                       tab.ent, cones_dev, n_fwd, n_phi, tab.rec, tab.hdr, tab.overflow);
+    return hipGetLastError();
+}
+
 hipError_t launch_cccs(const BeamletDev* beams, int n, int max_L, const ConeTable* cones_dev, int n_fwd,
                        int n_phi, bool exact_g, const float* terma, const float* dens_bev, float* partial,
-                       float* dose, uint64_t stride, hipStream_t stream, hipEvent_t* ev) {
+                       float* dose, uint64_t stride, hipStream_t stream, hipEvent_t* ev, const CccsTables* tab) {
     if (n <= 0) return hipSuccess;
     if (ev) (void)hipEventRecord(ev[0], stream);
-    if (n_fwd > 0) {
+    if (n_fwd > 0 && tab && n_phi <= kMaxMarchPhi) {
+        // the tables, then compute wavefronts only, reading their chunks from them
+        hipError_t et = launch_cccs_tables(*tab, cones_dev, n_fwd, n_phi, stream);
+        if (et != hipSuccess) return et;
+        const dim3 grid((unsigned)(n * n_fwd)), block(n_phi * kWave);
+        const size_t shm = tab_lds_bytes(n_phi);
+        if (exact_g)
+            hipLaunchKernelGGL(cccs_tab_kernel<true>, grid, block, shm, stream, beams, n, cones_dev, n_fwd, terma,
+                               dens_bev, partial, stride, tab->ent, tab->rec, tab->hdr);
+        else
+            hipLaunchKernelGGL(cccs_tab_kernel<false>, grid, block, shm, stream, beams, n, cones_dev, n_fwd, terma,
+                               dens_bev, partial, stride, tab->ent, tab->rec, tab->hdr);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    } else if (n_fwd > 0) {
         dim3 grid((unsigned)(n * n_fwd));
         // compute wavefronts + the march wavefront, within 1024 threads and 64 KiB of LDS
         if (n_phi <= kMaxMarchPhi) {

@@ -239,8 +239,11 @@ int fd_serve_start(fd_ctx* ctx, const fd_serve_opts* opts, fd_server** out) {
         set_error("n_slots must be in [0, 16], max_batch_beamlets and extent non-negative, threshold in [0, 1)");
         return FD_ERR_ARG;
     }
-    if (o.options & ~(FD_OPT_EXACT_G | FD_OPT_RPL_FP64 | FD_OPT_RPL_FP64_ALL | FD_OPT_GRAPH)) {
-        set_error("serving options: FD_OPT_EXACT_G, FD_OPT_RPL_FP64(_ALL) and FD_OPT_GRAPH only");
+    if (o.options & ~(FD_OPT_EXACT_G | FD_OPT_RPL_FP64 | FD_OPT_RPL_FP64_ALL | FD_OPT_GRAPH | FD_OPT_TABLES |
+                      FD_OPT_NO_TABLES) ||
+        ((o.options & FD_OPT_TABLES) && (o.options & FD_OPT_NO_TABLES))) {
+        set_error("serving options: FD_OPT_EXACT_G, FD_OPT_RPL_FP64(_ALL), FD_OPT_GRAPH and one of "
+                  "FD_OPT_TABLES / FD_OPT_NO_TABLES only");
         return FD_ERR_ARG;
     }
     if (o.n_slots == 0) o.n_slots = 3;

@@ -50,10 +50,16 @@ enum {
     FD_OPT_RPL_FP64_ALL = 8, /* FP64 radiological path on every ray */
     FD_OPT_GRAPH = 16,       /* batched paths (fd_dosemat_add_beam(s), serving) replay each batch as one
                                 captured HIP graph instead of separate launches */
-    FD_OPT_IEEE_DIV = 32     /* resampler: correct the refined hardware reciprocal on depths whose
+    FD_OPT_IEEE_DIV = 32,    /* resampler: correct the refined hardware reciprocal on depths whose
                                 significand is all ones even where the context's device check found
                                 it exact there (results are identical either way; no counterpart
                                 in the reference) */
+    FD_OPT_TABLES = 64,      /* batched paths: always walk the CCCS lattice once per march geometry
+                                (BEV pitch, longitudinal spacing) into tables the beamlets share;
+                                by default only when a batch has 4 or more beamlets per geometry
+                                (results are identical either way; FD_ERR_UNSUPPORTED beyond 10
+                                azimuths; no counterpart in the reference) */
+    FD_OPT_NO_TABLES = 128   /* batched paths: never use the tables (each CCCS workgroup marches) */
 };
 
 /* Value types of packed dose-loading matrices (fd_dosemat_pack, fd_spmm_create). */
