@@ -573,18 +573,21 @@ __device__ __forceinline__ CcLds carve_lds(float4* base, int n_phi) {
     return S;
 }
 
-// LDS of cccs_tab_kernel: one record area per compute wavefront (each finishes and reads only its
-// own), then the slabs, rho, cumulative length and per-azimuth doses as in carve_lds.  The record
-// areas come first, so the one read past a chunk's last record stays inside the allocation.
+// LDS of cccs_tab_kernel: two chunk buffers, each one record area per compute wavefront (kTabRec
+// float4: the chunk's records and one more, the read past a chunk's last) and the chunk header,
+// then the slabs, rho, cumulative length and per-azimuth doses as in carve_lds.  Chunk c's
+// buffer is filled by loads into LDS issued during chunk c - 1.
+constexpr int kTabRec = 2 * (kRecCap + 1);
 __host__ __device__ constexpr size_t tab_lds_bytes(int n_phi) {
-    return (size_t)n_phi * kRecStride * sizeof(float4) +
+    return (size_t)2 * n_phi * kTabRec * sizeof(float4) + (size_t)2 * kTabHdr * sizeof(int) +
            (size_t)(2 * kSlab + (3 + 2 + 2 * n_phi) * kNpix) * sizeof(float);
 }
 
 __device__ __forceinline__ CcLds carve_lds_tab(float4* base, int n_phi) {
     CcLds S{};
     S.rec = base;
-    float* f = reinterpret_cast<float*>(base + n_phi * kRecStride);
+    S.hdr = reinterpret_cast<int*>(base + 2 * n_phi * kTabRec);
+    float* f = reinterpret_cast<float*>(S.hdr + 2 * kTabHdr);
     S.slab = f;
     S.rho = f + 2 * kSlab;
     S.sC = S.rho + 3 * kNpix;
@@ -627,13 +630,16 @@ __device__ __forceinline__ void slab_regs(const CcBlock& K, int z, float& pf0, f
 }
 // End of a chunk, executed by every wavefront: store the prefetched slab, barrier, and after a
 // slice end the combine of the slice into P.  Returns false once the beamlet is finished.
+// DMA: this wavefront has loads into LDS in flight (cccs_tab_kernel's next chunk), complete before
+// the barrier (vector memory operations complete in issue order).
 __device__ __forceinline__ bool chunk_end(const CcBlock& K, const CcLds& S, int done, int sn, float pf0,
-                                          float pf1, int& s, int& si, int& written) {
+                                          float pf1, int& s, int& si, int& written, bool dma = false) {
     if (done && sn < K.L && K.prefetch) {
         const int i0 = K.tid, i1 = K.tid + K.nthreads;
         if (i0 < 2 * kNpix) store_slab(K, S, si + 1, i0, pf0);
         if (K.two && i1 < 2 * kNpix) store_slab(K, S, si + 1, i1, pf1);
     }
+    if (dma) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     if (!done) return true;
     // slice s: rho sum over azimuths (in azimuth order) of the dose in units of geometric length /
@@ -968,6 +974,29 @@ struct TabRead {
     float lim_min, inv_sad, bsx, bsy, ls;
 };
 
+// 16 bytes per lane from global address base + off (base wave-uniform, off < 2^32 per lane) into
+// LDS at lds + 16 lane (global_load_lds_dwordx4; lds wave-uniform).  Written as asm so that the
+// compiler, which does not see the LDS write, adds no wait for it before the wavefront's other LDS
+// reads; the caller waits (vmcnt) before the barrier after which the data is read.  The
+// instruction has no register result.
+__device__ __forceinline__ void load_to_lds16(const void* base, uint32_t off, void* lds) {
+    const uint32_t a = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) void*)lds;
+    asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(off), "s"(base), "s"(a) : "m0");
+}
+
+// Chunk `chunk` of a march table into LDS: by wavefront 0, the chunk's header (4 lanes x 16 bytes)
+// into `hdr`, and the kRecCap records of this azimuth's stream from record `ofs` on (64 lanes x 16
+// bytes; indices clamped into the stream, what lies past the chunk's records is never read) into
+// `rec`.
+__device__ __forceinline__ void tab_load_chunk(const TabRead& TR, uint32_t chunk, uint32_t ofs, float4* rec, int* hdr,
+                                               int wave) {
+    const uint32_t lane = threadIdx.x & 63;
+    if (wave == 0 && lane < 4)
+        load_to_lds16(TR.hdr + (uint64_t)min(chunk, TR.chunks - 1) * kTabHdr, 16 * lane, hdr);
+    const uint32_t i = min(2 * ofs + lane, 2 * TR.cap - 1);
+    load_to_lds16(TR.rec, 16 * i, rec);
+}
+
 // Compute wavefront of azimuth `wave`: lane l owns voxels (x = l & 15, y = 4 (l >> 4) + r).  The
 // X states are kept pre-scaled by A/a and B/b, and the dose and cumulative length are summed in
 // units of the geometric length s (p = s rho), multiplied by rho in the combine.  TAB: the chunks
@@ -998,6 +1027,10 @@ __device__ __forceinline__ void cccs_compute(const CcBlock& K, const CcLds& S, i
 #pragma unroll
     for (int r = 0; r < kSlots; r++) fiy[r] = (float)(4 * g + r);
 
+    if (TAB) {
+        tab_load_chunk(TR, 0, 0, S.rec + wave * kTabRec, S.hdr, wave);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    }
     __syncthreads();  // chunk 0 and slab 0 are in LDS
     int s = 0, buf = 0, si = 0, written = 0;  // si: ordinal of the slice being processed
     uint32_t tc = 0, tofs = 0;                 // TAB: chunk, records of my azimuth before it
@@ -1007,31 +1040,32 @@ __device__ __forceinline__ void cccs_compute(const CcBlock& K, const CcLds& S, i
         int done, sn, cnt;
         const float4* rp;
         if (TAB) {
-            // this chunk's header, and my azimuth's records finished with the beamlet terms into my
-            // LDS records (the arithmetic of the march wavefront's phase 2, same operation order)
-            const int* h = TR.hdr + (uint64_t)min(tc, TR.chunks - 1) * kTabHdr;
+            // this chunk's header and my azimuth's records are in LDS (loaded during the previous
+            // chunk); the records are finished in place with the beamlet terms (the arithmetic of the
+            // march wavefront's phase 2, same operation order), then the next chunk's loads start
+            const int* h = S.hdr + (tc & 1) * kTabHdr;
             done = h[kTabHdr - 2];
             sn = h[kTabHdr - 1];
-            cnt = min(max(h[wave], 0), kRecCap);
-            float4* my = S.rec + wave * kRecStride;
-            // every lane loads (a clamped index keeps the spare lanes inside the stream)
-            const uint32_t idx = min(tofs + (uint32_t)lane, TR.cap - 1);
-            const float4 g0 = TR.rec[2 * (uint64_t)idx], g1 = TR.rec[2 * (uint64_t)idx + 1];
-            // g0 = (dzs, z1, x0 - 8, y0 - 8), g1 = (bsx dx, bsy dy, w, gx | gy << 16)
-            const float fe = fmaf(g0.y, TR.ls, TR.lim_min) * TR.inv_sad;
-            const float df = g0.x * TR.inv_sad;
-            const float gx = TR.bsx * df, gy = TR.bsy * df;
-            const int gxy = __float_as_int(g1.w);
-            float hc;
-            const float4 hr = heron_record(gx, gy, fmaf(gx, g0.z, g1.x * fe), fmaf(gy, g0.w, g1.y * fe), g0.x * g0.x, hc);
+            cnt = min(max(__builtin_amdgcn_readfirstlane(h[wave]), 0), kRecCap);
+            float4* my = S.rec + ((tc & 1) * n_phi + wave) * kTabRec;
             if (lane < cnt) {
-                my[2 * lane] = hr;
+                const float4 g0 = my[2 * lane], g1 = my[2 * lane + 1];
+                // g0 = (dzs, z1, x0 - 8, y0 - 8), g1 = (bsx dx, bsy dy, w, gx | gy << 16)
+                const float fe = fmaf(g0.y, TR.ls, TR.lim_min) * TR.inv_sad;
+                const float df = g0.x * TR.inv_sad;
+                const float gx = TR.bsx * df, gy = TR.bsy * df;
+                const int gxy = __float_as_int(g1.w);
+                float hc;
+                my[2 * lane] = heron_record(gx, gy, fmaf(gx, g0.z, g1.x * fe), fmaf(gy, g0.w, g1.y * fe),
+                                            g0.x * g0.x, hc);
                 my[2 * lane + 1] = make_float4(hc, g1.z, __int_as_float((int)(short)(gxy & 0xFFFF)),
                                                __int_as_float(gxy >> 16));
             }
             rp = my;
             tofs += (uint32_t)cnt;
             tc++;
+            tab_load_chunk(TR, tc, tofs, S.rec + ((tc & 1) * n_phi + wave) * kTabRec, S.hdr + (tc & 1) * kTabHdr,
+                           wave);
         } else {
             done = S.hdr[buf * kHdr + kMaxPhi];
             sn = S.hdr[buf * kHdr + kMaxPhi + 1];
@@ -1203,7 +1237,7 @@ __device__ __forceinline__ void cccs_compute(const CcBlock& K, const CcLds& S, i
                 if (CUMU) S.sC[(si & 1) * kNpix + vbase + 16 * r] = Cs[r];
             }
         }
-        if (!chunk_end(K, S, done, sn, pf0, pf1, s, si, written)) break;
+        if (!chunk_end(K, S, done, sn, pf0, pf1, s, si, written, TAB)) break;
         fresh = done != 0;
         buf ^= 1;
     }
