@@ -34,6 +34,7 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FLOPS_PER_SEGMENT = 40  # algorithmic flops of one (ray, segment) CCCS step, exp counted as 1
+N_CU, CLOCK_HZ = 256, 2.4e9  # MI355X compute units and peak engine clock
 TOL = 0.005
 
 
@@ -63,16 +64,21 @@ def cpu_model() -> str:
         return ""
 
 
-def profiled_traffic(name: str):
-    """HBM bytes per launch of a kernel from a committed rocprofv3 counter summary (FETCH_SIZE +
-    WRITE_SIZE, scaled by the same counters on a copy of known size), or None."""
+def profiled_value(name: str, key: str):
+    """A `key: value` line of a committed rocprofv3 summary under profiles/, or None."""
     import re
 
     path = ROOT / "profiles" / name
     if not path.exists():
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", path.read_text())
+    m = re.search(rf"{key}:\s*([0-9.eE+]+)", path.read_text())
     return int(float(m.group(1))) if m else None
+
+
+def profiled_traffic(name: str):
+    """HBM bytes per launch of a kernel from a committed rocprofv3 counter summary (FETCH_SIZE +
+    WRITE_SIZE, scaled by the same counters on a copy of known size), or None."""
+    return profiled_value(name, "traffic_bytes_per_launch")
 
 
 # ---------------------------------------------------------------------------- oracle side (checker)
@@ -238,6 +244,9 @@ def leg_headline(args, fd, synth, workload, torch, np):
     launches = {k: int(round(float(np.mean(v)))) for k, v in kn.items()}
     nseg = int(nseg_of(ctx, fd, arr).sum())
     cccs_launch_ms = kstep["cccs"] / launches["cccs"]
+    # batches of many beamlets per march geometry run the table-fed CCCS kernel (its tables are
+    # timed in their own slot, "cccs_tables")
+    cccs_name = "cccs_tab_kernel" if "cccs_tables" in kstep else "cccs_kernel"
     achieved = FLOPS_PER_SEGMENT * nseg / (kstep["cccs"] * 1e-3) / 1e12
     res = {
         "value": n * args.steps / elapsed,
@@ -251,22 +260,31 @@ def leg_headline(args, fd, synth, workload, torch, np):
         "kernel_launches_per_step": launches,
         "roofline": {
             "bound": "valu",
-            "kernel": "cccs_kernel",
+            "kernel": cccs_name,
             "achieved": round(achieved, 3),
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-            "traffic": profiled_traffic("cccs_kernel_batched4pi_traffic.md"),
+            "traffic": profiled_traffic(f"{cccs_name}_batched4pi_traffic.md"),
             "algorithmic": f"{FLOPS_PER_SEGMENT} flop x {nseg // launches['cccs']} (ray, segment) steps per launch "
                            f"(average of {launches['cccs']} launches per step, {nseg} per step); "
                            f"average launch {cccs_launch_ms:.3f} ms",
-            # the published FP32 peak counts packed (two-wide) FMAs, which this build does not emit
-            "peak_non_packed": FP32_PEAK_TFLOPS / 2,
-            "frac_of_non_packed": round(achieved / (FP32_PEAK_TFLOPS / 2), 4),
+            # the vector instructions the kernel issues (counter summary of the same workload): wave64
+            # instructions per CU per clock at the bench's own launch time
+            "issue": issue_block(f"{cccs_name}_batched4pi_counters.md", cccs_launch_ms),
         },
     }
     # the last step's matrices for the accuracy check, kept with the beamlet store
     return res, (cfg, ctx, store, dm, arr, flu, cnt)
+
+
+def issue_block(name: str, launch_ms: float):
+    valu = profiled_value(name, "valu_instr_per_launch")
+    if valu is None:
+        return None
+    return {"valu_instr_per_launch": valu, "trans_instr_per_launch": profiled_value(name, "trans_instr_per_launch"),
+            "valu_per_cu_cycle": round(valu / (N_CU * CLOCK_HZ * launch_ms * 1e-3), 3),
+            "source": f"profiles/{name} (SQ_INSTS_VALU), {N_CU} CUs at {CLOCK_HZ / 1e9:.1f} GHz"}
 
 
 def leg_plain_4pi(fd, torch, np, state):

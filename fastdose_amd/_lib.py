@@ -31,7 +31,8 @@ FD_VAL_F32 = 0
 FD_VAL_BF16 = 1
 FD_VAL_FP16 = 2
 
-TIMING_SLOTS = ("terma", "cccs", "cone_sum", "resample_count", "resample_values", "resample_compact")
+TIMING_SLOTS = ("terma", "cccs", "cone_sum", "resample_count", "resample_values", "resample_compact",
+                "cccs_tables")
 
 MAX_TABLE = 24
 

@@ -251,6 +251,15 @@ int reserve_cccs(const fd_ctx* ctx, Workspace& w, int n, uint64_t stride) {
 int run_cccs(const fd_ctx* ctx, Workspace& w, int options, int n, int max_L, const float* terma, const float* dens,
              float* dose, uint64_t stride, hipStream_t stream, const CccsTables* tab) {
     FD_TRY(reserve_cccs(ctx, w, n, stride));
+    if (tab) {
+        w.mark(FD_T_CCCS_TABLES, 0, stream);
+        const hipError_t et = launch_cccs_tables(*tab, ctx->cones_dev, ctx->n_fwd, ctx->cones.n_phi, stream);
+        w.mark(FD_T_CCCS_TABLES, 1, stream);
+        if (et != hipSuccess) {
+            set_error(std::string("cccs tables launch: ") + hipGetErrorString(et));
+            return FD_ERR_HIP;
+        }
+    }
     hipEvent_t evs[3];
     const bool t = w.timing && w.ev[0];
     if (t) {

@@ -46,9 +46,9 @@ int cccs_chunk_steps();  // lattice steps of one azimuth per chunk at most
 // CCCS over the forward cones (DoseComp.cu:217-392): per-cone partial BEV doses
 // partial[(b * n_fwd + t) * stride + s * 256 + p], then their sum into dose.
 // ev (nullable): events recorded before the CCCS kernel, between it and the cone sum, and after.
-// tab (nullable, n_phi <= cccs_table_max_phi()): builds the beamlets' march tables first (its
-// entries already on the device, beams[b].geom indexing them) and runs the CCCS from them; the
-// results are those of the march wavefront.  The events then time the tables with the CCCS.
+// tab (nullable, n_phi <= cccs_table_max_phi()): the beamlets' march tables, built on this stream
+// beforehand (launch_cccs_tables), beams[b].geom indexing them; the CCCS reads its chunks from
+// them, with the results of the march wavefront.
 hipError_t launch_cccs(const BeamletDev* beams, int n, int max_L, const ConeTable* cones_dev, int n_fwd,
                        int n_phi, bool exact_g, const float* terma, const float* dens_bev, float* partial,
                        float* dose, uint64_t stride, hipStream_t stream, hipEvent_t* ev = nullptr,

@@ -1390,9 +1390,7 @@ hipError_t launch_cccs(const BeamletDev* beams, int n, int max_L, const ConeTabl
     if (n <= 0) return hipSuccess;
     if (ev) (void)hipEventRecord(ev[0], stream);
     if (n_fwd > 0 && tab && n_phi <= kMaxMarchPhi) {
-        // the tables, then compute wavefronts only, reading their chunks from them
-        hipError_t et = launch_cccs_tables(*tab, cones_dev, n_fwd, n_phi, stream);
-        if (et != hipSuccess) return et;
+        // compute wavefronts only, reading their chunks from the tables
         const dim3 grid((unsigned)(n * n_fwd)), block(n_phi * kWave);
         const size_t shm = tab_lds_bytes(n_phi);
         if (exact_g)

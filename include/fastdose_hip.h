@@ -77,7 +77,8 @@ enum {
     FD_T_RESAMPLE_COUNT = 3,
     FD_T_RESAMPLE_VALUES = 4,
     FD_T_RESAMPLE_COMPACT = 5,
-    FD_T_COUNT = 6
+    FD_T_CCCS_TABLES = 6,  /* the march tables of a batch (FD_OPT_TABLES) */
+    FD_T_COUNT = 7
 };
 
 /* DENSITY_h geometry (density.cuh:9-28): voxel size [cm], volume dims (x fastest), bbox. */
