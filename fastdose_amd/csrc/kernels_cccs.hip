@@ -522,10 +522,13 @@ static_assert(sizeof(StepRec) == 32, "StepRec is two 16-byte LDS accesses");
 // w: [0,5) shift of the midpoint voxel along x (two's complement), [5,10) along y, [10,14) mx & 15,
 //    [14,18) my & 15, bit 18 + [19,23) voxel column whose rays crossed the x edge, bit 23 + [24,28)
 //    voxel row whose rays crossed the y edge, bit 28: a step of more than one cell (test each ray),
-//    bit 29: the previous step's column reset is done by this step's x shift (zero fill).
+//    bit 29: the previous step's column reset is done by this step's x shift (zero fill), bits 30 /
+//    31: the x / y shift is one voxel.
 constexpr int kRecCap = 32;       // steps per azimuth per chunk
 constexpr int kHdr = kMaxPhi + 2; // per chunk: step count of each azimuth, slice done, next slice
 constexpr uint32_t kColReset = 1u << 18, kRowReset = 1u << 23, kGeneralReset = 1u << 28, kZeroFillX = 1u << 29;
+// bit 30 / 31: the shift along x / y is exactly one voxel (its direction is the field's sign bit, 4 / 9)
+constexpr uint32_t kUnitX = 1u << 30, kUnitY = 1u << 31;
 constexpr uint32_t kShiftBits = 0x3FFu, kResetBits = kColReset | kRowReset | kGeneralReset;
 constexpr int kMaxMarchPhi = 10;  // azimuths of the march kernel (64 KiB of LDS); more: inline kernel
 constexpr int kMaxPhiPerWave1 = 16;  // self-marching kernel: one azimuth per wavefront up to 16, then two
@@ -781,7 +784,8 @@ __device__ __forceinline__ bool march_chunk(MarchLane& M, const MarchConst& C, i
         const int fbx = (int)floorf(p0.x), fby = (int)floorf(p0.y);
         const int fnx = (int)floorf(p1.x), fny = (int)floorf(p1.y);
         uint32_t w = (uint32_t)(ddx & 31) | ((uint32_t)(ddy & 31) << 5) | ((uint32_t)(mx & 15) << 10) |
-                     ((uint32_t)(my & 15) << 14);
+                     ((uint32_t)(my & 15) << 14) | (ddx == 1 || ddx == -1 ? kUnitX : 0u) |
+                     (ddy == 1 || ddy == -1 ? kUnitY : 0u);
         // rays that cross the lateral BEV edge restart with X = 0 (DoseComp.cu:372-377); a step
         // of at most one cell per axis crosses with one ray column and one ray row at most
         if (abs(fnx - fbx) <= 1 && abs(fny - fby) <= 1) {
@@ -1096,55 +1100,59 @@ __device__ __forceinline__ void cccs_compute(const CcBlock& K, const CcLds& S, i
             // trees with one scalar test each
             if (w & kShiftBits) {
                 // rotate the X state (and my rays' x offsets) to follow the midpoint voxel
-                if (ddx == 1) {
-                    if (w & kZeroFillX) {  // with the previous step's reset of column 15
+                if (w & kUnitX) {
+                    if (w & 0x10u) {  // ddx = -1
+                        if (w & kZeroFillX) {  // with the previous step's reset of column 0
 #pragma unroll
-                        for (int r = 0; r < kSlots; r++) {
-                            XA[r] = row_from_left_zero(XA[r]);
-                            XB[r] = row_from_left_zero(XB[r]);
-                        }
-                    } else {
+                            for (int r = 0; r < kSlots; r++) {
+                                XA[r] = row_from_right_zero(XA[r]);
+                                XB[r] = row_from_right_zero(XB[r]);
+                            }
+                        } else {
 #pragma unroll
-                        for (int r = 0; r < kSlots; r++) {
-                            XA[r] = row_from_left(XA[r]);
-                            XB[r] = row_from_left(XB[r]);
+                            for (int r = 0; r < kSlots; r++) {
+                                XA[r] = row_from_right(XA[r]);
+                                XB[r] = row_from_right(XB[r]);
+                            }
                         }
+                        fix = row_from_right(fix);
+                    } else {  // ddx = +1
+                        if (w & kZeroFillX) {  // with the previous step's reset of column 15
+#pragma unroll
+                            for (int r = 0; r < kSlots; r++) {
+                                XA[r] = row_from_left_zero(XA[r]);
+                                XB[r] = row_from_left_zero(XB[r]);
+                            }
+                        } else {
+#pragma unroll
+                            for (int r = 0; r < kSlots; r++) {
+                                XA[r] = row_from_left(XA[r]);
+                                XB[r] = row_from_left(XB[r]);
+                            }
+                        }
+                        fix = row_from_left(fix);
                     }
-                    fix = row_from_left(fix);
-                } else if (ddx == -1) {
-                    if (w & kZeroFillX) {  // with the previous step's reset of column 0
-#pragma unroll
-                        for (int r = 0; r < kSlots; r++) {
-                            XA[r] = row_from_right_zero(XA[r]);
-                            XB[r] = row_from_right_zero(XB[r]);
-                        }
-                    } else {
-#pragma unroll
-                        for (int r = 0; r < kSlots; r++) {
-                            XA[r] = row_from_right(XA[r]);
-                            XB[r] = row_from_right(XB[r]);
-                        }
-                    }
-                    fix = row_from_right(fix);
-                } else if (ddx != 0) {
+                } else if (w & 0x1Fu) {
                     shift_x(XA, XB, ddx);
                     fix = (float)((vx - (int)((w >> 10) & 15)) & 15);
                 }
-                if (ddy == 1) {
-                    const float ta = wave_from(XA[kSlots - 1], up_lane);
-                    const float tb = wave_from(XB[kSlots - 1], up_lane);
-                    const float tz = wave_from(fiy[kSlots - 1], up_lane);
-                    slots_down(XA, ta);
-                    slots_down(XB, tb);
-                    slots_down(fiy, tz);
-                } else if (ddy == -1) {
-                    const float ta = wave_from(XA[0], down_lane);
-                    const float tb = wave_from(XB[0], down_lane);
-                    const float tz = wave_from(fiy[0], down_lane);
-                    slots_up(XA, ta);
-                    slots_up(XB, tb);
-                    slots_up(fiy, tz);
-                } else if (ddy != 0) {
+                if (w & kUnitY) {
+                    if (w & 0x200u) {  // ddy = -1
+                        const float ta = wave_from(XA[0], down_lane);
+                        const float tb = wave_from(XB[0], down_lane);
+                        const float tz = wave_from(fiy[0], down_lane);
+                        slots_up(XA, ta);
+                        slots_up(XB, tb);
+                        slots_up(fiy, tz);
+                    } else {  // ddy = +1
+                        const float ta = wave_from(XA[kSlots - 1], up_lane);
+                        const float tb = wave_from(XB[kSlots - 1], up_lane);
+                        const float tz = wave_from(fiy[kSlots - 1], up_lane);
+                        slots_down(XA, ta);
+                        slots_down(XB, tb);
+                        slots_down(fiy, tz);
+                    }
+                } else if (w & 0x3E0u) {
                     shift_y3(XA, XB, fiy, ddy, up_lane, down_lane);
                 }
             }
