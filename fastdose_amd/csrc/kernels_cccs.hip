@@ -186,14 +186,17 @@ __device__ __forceinline__ int crossing_offset(int fb, int fn) {
 // is within (q - S0)^2 / (8 S0 q) of sqrt(q) and never below it: 2.2e-5 relative at most for
 // 1.25 mm pixels at 100 cm, 9e-5 for 2.5 mm.  With k = 1 / sqrt(2 s0) the ray terms are carried
 // scaled by k and the constant c = k^2 dzs^2 + s0 / 2, so s = (k ddy)^2 + ((k ddx)^2 + c): two FMAs
-// on values the step computes anyway.  A step of length 0 (S0 = 0, hence every ray 0) gets k = c = 0.
-// Returns the step's record (gx, gy, axc, ayc) k and c.
+// on values the step computes anyway.  s0 and k come from the hardware square root and reciprocal
+// square root (within an ulp or so, which moves s by about 1e-7 relative).  A step with S0 < 1e-30
+// (every ray shorter than 1e-15 cm; 0 for S0 = 0) gets k = c = 0, length 0.  Returns the step's
+// record (gx, gy, axc, ayc) k and c.
 __device__ __forceinline__ float4 heron_record(float gx, float gy, float axc, float ayc, float dz2, float& c) {
     const float cx = fmaf(gx, 7.5f, axc), cy = fmaf(gy, 7.5f, ayc);
     const float S0 = fmaf(cx, cx, fmaf(cy, cy, dz2));
-    const float s0 = sqrtf(S0);
-    const float k = S0 > 0.0f ? rsqrtf(2.0f * s0) : 0.0f;
-    c = S0 > 0.0f ? fmaf(k * k, dz2, 0.5f * s0) : 0.0f;
+    const bool pos = S0 > 1e-30f;
+    const float s0 = __builtin_amdgcn_sqrtf(pos ? S0 : 1.0f);
+    const float k = pos ? __builtin_amdgcn_rsqf(2.0f * s0) : 0.0f;
+    c = pos ? fmaf(k * k, dz2, 0.5f * s0) : 0.0f;
     return make_float4(gx * k, gy * k, axc * k, ayc * k);
 }
 
