@@ -95,6 +95,7 @@ struct Workspace {
     DevBuf<float> partial;
     DevBuf<float> fluence, terma, dens_bev, dose_bev;
     DevBuf<uint32_t> cand_count, nnz_count;
+    DevBuf<uint2> cand_rows;      // per segment: rows [y_lo, y_hi) holding candidates
     DevBuf<uint64_t> cand_off, nnz_off;
     DevBuf<uint64_t> tmp_cols;
     DevBuf<float> tmp_vals;

@@ -51,6 +51,7 @@ void Workspace::release() {
     dens_bev.release();
     dose_bev.release();
     cand_count.release();
+    cand_rows.release();
     nnz_count.release();
     cand_off.release();
     nnz_off.release();
@@ -83,9 +84,9 @@ int Workspace::fold(int slot) {
 
 size_t Workspace::footprint() const {
     return desc.bytes() + partial.bytes() + fluence.bytes() + terma.bytes() + dens_bev.bytes() + dose_bev.bytes() +
-           cand_count.bytes() + nnz_count.bytes() + cand_off.bytes() + nnz_off.bytes() + tmp_cols.bytes() +
-           tmp_vals.bytes() + rowoff.bytes() + out_cols.bytes() + out_vals.bytes() + flags.bytes() + rowmax.bytes() +
-           tab_ent.bytes() + tab_rec.bytes() + tab_hdr.bytes();
+           cand_count.bytes() + cand_rows.bytes() + nnz_count.bytes() + cand_off.bytes() + nnz_off.bytes() +
+           tmp_cols.bytes() + tmp_vals.bytes() + rowoff.bytes() + out_cols.bytes() + out_vals.bytes() + flags.bytes() +
+           rowmax.bytes() + tab_ent.bytes() + tab_rec.bytes() + tab_hdr.bytes();
 }
 
 int check_beams(const fd_beam* beams, int n) {
@@ -194,7 +195,7 @@ size_t batch_footprint(const fd_ctx* ctx, const fd_beam* beams, int n) {
     for (int i = 0; i < n; i++) cand += frustum_voxels(beams[i], ctx->vol);
     const size_t segs = (size_t)n * ctx->vol.dim[2];
     return (size_t)n * stride * sizeof(float) * (3 + std::max(ctx->n_fwd, 4)) + (size_t)n * kNpix * sizeof(float) +
-           (size_t)(2.0 * cand) * (2 * sizeof(uint64_t) + 2 * sizeof(float)) + segs * 24 + (size_t)n * 128;
+           (size_t)(2.0 * cand) * (2 * sizeof(uint64_t) + 2 * sizeof(float)) + segs * 32 + (size_t)n * 128;
 }
 
 int upload_beams(Workspace& w, const fd_beam* beams, int n, uint64_t stride, hipStream_t stream) {
@@ -289,6 +290,7 @@ namespace {
 int reserve_segments(const fd_ctx* ctx, Workspace& w, int n) {
     const size_t segs = (size_t)n * ctx->vol.dim[2];
     FD_TRY(w.cand_count.reserve(segs));
+    FD_TRY(w.cand_rows.reserve(segs));
     FD_TRY(w.nnz_count.reserve(segs));
     FD_TRY(w.cand_off.reserve(segs + 1));
     FD_TRY(w.nnz_off.reserve(segs + 1));
@@ -301,7 +303,7 @@ int reserve_segments(const fd_ctx* ctx, Workspace& w, int n) {
 int launch_count(const fd_ctx* ctx, Workspace& w, int n, hipStream_t stream) {
     const size_t segs = (size_t)n * ctx->vol.dim[2];
     w.mark(FD_T_RESAMPLE_COUNT, 0, stream);
-    hipError_t e = launch_resample_count(w.desc.ptr, n, ctx->vol, w.cand_count.ptr, stream);
+    hipError_t e = launch_resample_count(w.desc.ptr, n, ctx->vol, w.cand_count.ptr, w.cand_rows.ptr, stream);
     if (e == hipSuccess) e = launch_scan_u32(w.cand_count.ptr, segs, w.cand_off.ptr, w.cand_off.ptr + segs, w.scan_tmp.ptr, stream);
     w.mark(FD_T_RESAMPLE_COUNT, 1, stream);
     if (e != hipSuccess) {
@@ -319,7 +321,7 @@ int launch_values(const fd_ctx* ctx, Workspace& w, int n, const float* dose, flo
     hipError_t e = launch_fill_u32(w.flags.ptr, 2, 0u, stream);
     if (e == hipSuccess && thr > 0.0f) e = launch_fill_u32(w.rowmax.ptr, (uint64_t)n, 0u, stream);
     if (e == hipSuccess)
-        e = launch_resample_values(w.desc.ptr, n, ctx->vol, extent, dose, w.cand_off.ptr, cap, w.flags.ptr,
+        e = launch_resample_values(w.desc.ptr, n, ctx->vol, extent, dose, w.cand_off.ptr, w.cand_rows.ptr, cap, w.flags.ptr,
                                    w.tmp_cols.ptr, w.tmp_vals.ptr, w.nnz_count.ptr,
                                    thr > 0.0f ? w.rowmax.ptr : nullptr,
                                    ctx->rcp_exact && !(ctx->options & FD_OPT_IEEE_DIV), stream);

@@ -54,18 +54,19 @@ hipError_t launch_cccs(const BeamletDev* beams, int n, int max_L, const ConeTabl
                        float* dose, uint64_t stride, hipStream_t stream, hipEvent_t* ev = nullptr,
                        const CccsTables* tab = nullptr);
 
-// Resampling into CSR rows.  Stage 1 counts candidate voxels per (beamlet, z-plane).
+// Resampling into CSR rows.  Stage 1 counts candidate voxels per (beamlet, z-plane) segment and
+// records the rows [y_lo, y_hi) of the segment that hold them in cand_rows ((0, 0) when none).
 hipError_t launch_resample_count(const BeamletDev* beams, int n, const fd_volume& vol, uint32_t* cand_count,
-                                 hipStream_t stream);
+                                 uint2* cand_rows, hipStream_t stream);
 // Stage 2: values of every candidate, nonzeros compacted in voxel order into tmp_cols/tmp_vals
 // at cand_off[b * Z + z]; nonzero counts into nnz_count.  A segment whose candidates would end
 // beyond `cap` entries writes nothing, gets a zero count and sets overflow[0] = 1.
 // rowmax_bits (nullable, zeroed by the caller): per-beamlet maximum value, as float bits.
 // rcp_exact: launch_rcp_check found no mismatch on this device (see there).
 hipError_t launch_resample_values(const BeamletDev* beams, int n, const fd_volume& vol, float extent,
-                                  const float* dose_bev, const uint64_t* cand_off, uint64_t cap, uint32_t* overflow,
-                                  uint64_t* tmp_cols, float* tmp_vals, uint32_t* nnz_count, uint32_t* rowmax_bits,
-                                  bool rcp_exact, hipStream_t stream);
+                                  const float* dose_bev, const uint64_t* cand_off, const uint2* cand_rows,
+                                  uint64_t cap, uint32_t* overflow, uint64_t* tmp_cols, float* tmp_vals,
+                                  uint32_t* nnz_count, uint32_t* rowmax_bits, bool rcp_exact, hipStream_t stream);
 // Adds to *mismatches (zeroed by the caller) the number of floats b with exponent in
 // [exp0, exp0 + n_exp) for which the resampler's reciprocal (hardware reciprocal + one Newton
 // step) differs from the IEEE quotient 1 / b.

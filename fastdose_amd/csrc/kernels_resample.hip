@@ -126,8 +126,9 @@ constexpr int kCountPlanes = 8;
 
 __global__ __launch_bounds__(kRowThreads) void resample_count_kernel(const BeamletDev* __restrict__ beams, int X,
                                                                      int Y, int Z, float vsx, float vsy, float vsz,
-                                                                     uint32_t* __restrict__ cand_count) {
-    __shared__ uint32_t s_cnt[kCountPlanes];
+                                                                     uint32_t* __restrict__ cand_count,
+                                                                     uint2* __restrict__ cand_rows) {
+    __shared__ uint32_t s_cnt[kCountPlanes], s_ylo[kCountPlanes], s_yhi[kCountPlanes];
     const int z0 = blockIdx.x * kCountPlanes, b = blockIdx.y;
     const BeamletDev& B = beams[b];
     const double rho = rho_of(vsx, vsy, vsz);
@@ -152,17 +153,31 @@ __global__ __launch_bounds__(kRowThreads) void resample_count_kernel(const Beaml
     }
     const int za = max(z0, (int)floor((zmin - 1e-3) / (double)vsz - 0.5));
     const int zb = min(min(z0 + kCountPlanes, Z), (int)ceil((zmax + 1e-3) / (double)vsz - 0.5) + 1);
-    if (threadIdx.x < kCountPlanes) s_cnt[threadIdx.x] = 0u;
+    if (threadIdx.x < kCountPlanes) {
+        s_cnt[threadIdx.x] = 0u;
+        s_ylo[threadIdx.x] = 0xFFFFFFFFu;
+        s_yhi[threadIdx.x] = 0u;
+    }
     __syncthreads();
     const int np = max(zb - za, 0);
     for (int i = threadIdx.x; i < np * Y; i += blockDim.x) {
         const int p = i / Y, y = i - p * Y;
         int x0, x1;
         row_interval(B, y, za + p, X, vsx, vsy, vsz, rho, x0, x1);
-        if (x1 > x0) atomicAdd(&s_cnt[za - z0 + p], (uint32_t)(x1 - x0));
+        if (x1 > x0) {
+            atomicAdd(&s_cnt[za - z0 + p], (uint32_t)(x1 - x0));
+            atomicMin(&s_ylo[za - z0 + p], (uint32_t)y);
+            atomicMax(&s_yhi[za - z0 + p], (uint32_t)y + 1u);
+        }
     }
     __syncthreads();
-    if (threadIdx.x < min(kCountPlanes, Z - z0)) cand_count[(size_t)b * Z + z0 + threadIdx.x] = s_cnt[threadIdx.x];
+    if (threadIdx.x < min(kCountPlanes, Z - z0)) {
+        const size_t seg = (size_t)b * Z + z0 + threadIdx.x;
+        const uint32_t cnt = s_cnt[threadIdx.x];
+        cand_count[seg] = cnt;
+        // rows [y_lo, y_hi) of the plane that hold candidates (the values pass walks only those)
+        cand_rows[seg] = cnt ? make_uint2(s_ylo[threadIdx.x], s_yhi[threadIdx.x]) : make_uint2(0u, 0u);
+    }
 }
 
 // floor(v) as an integer (v_cvt_flr_i32_f32: out-of-range values saturate)
@@ -176,9 +191,14 @@ __device__ __forceinline__ int floor_int(float v) {
 // the device's reciprocal followed by one Newton step equals the IEEE quotient for every input
 // (checked on the device when a context is created); otherwise the one significand where the step
 // can tie is corrected.
+// The 5^3 samples are summed as five partial sums, one per x-sample plane ii (25 samples each, in
+// (jj, kk) order), added in ii order: p0 + p1 + p2 + p3 + p4.  k lanes (1, 2, 3 or 5, wave-uniform)
+// share a voxel: in round r lane sub takes plane ii = r k + sub, and the group's first lane adds
+// the round's partial sums in ii order, so the sum has the same rounding for every k.  Returns
+// the value on the group's first lane, 0 on the others.
 template <bool RCP_EXACT>
 __device__ float voxel_value(const BeamletDev& B, const float* __restrict__ dose, int x, int y, int z, float vsx,
-                             float vsy, float vsz, float extent, float half_diag, float half_sdiag) {
+                             float vsy, float vsz, float extent, float half_diag, float half_sdiag, int k, int sub) {
     const float* M = B.Rf;
     const float s0 = B.src[0], s1 = B.src[1], s2 = B.src[2];
     // super-voxel test (d_superVoxelInterp, IMRTgeom.cu:471-516)
@@ -205,11 +225,10 @@ __device__ float voxel_value(const BeamletDev& B, const float* __restrict__ dose
     const float half = 0.5f * kFmap;
     const float* __restrict__ D = dose + B.bev_off;
     // per-axis sample coordinates minus source
-    float qx[kResampleSS], qy[kResampleSS], qz[kResampleSS];
+    float qy[kResampleSS], qz[kResampleSS];
 #pragma unroll
     for (int i = 0; i < kResampleSS; i++) {
         const float o = (i + 0.5f) * inv_ss;
-        qx[i] = ((float)x + o) * vsx - s0;
         qy[i] = ((float)y + o) * vsy - s1;
         qz[i] = ((float)z + o) * vsz - s2;
     }
@@ -228,50 +247,69 @@ __device__ float voxel_value(const BeamletDev& B, const float* __restrict__ dose
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc((void*)D, (short)0, (int)(Lu * kNpix * sizeof(float)), 0x00020000);
     constexpr uint32_t kOutside = 0x80000000u;  // byte offset beyond any beamlet's range
-    float acc = 0.0f;
-    for (int ii = 0; ii < kResampleSS; ii++) {
-        const float px = M[0] * qx[ii], py = M[3] * qx[ii], pz = M[6] * qx[ii];
-        for (int jj = 0; jj < kResampleSS; jj++) {
-            const float ax = fmaf(M[1], qy[jj], px), ay = fmaf(M[4], qy[jj], py), az = fmaf(M[7], qy[jj], pz);
-            float val[kResampleSS];
+    const int rounds = (kResampleSS + k - 1) / k;
+    const int lane = threadIdx.x & 63;
+    float sum = 0.0f;
+    for (int r = 0; r < rounds; r++) {
+        const int ii = r * k + sub;
+        float acc = 0.0f;
+        if (ii < kResampleSS) {
+            const float qx = ((float)x + ((float)ii + 0.5f) * inv_ss) * vsx - s0;
+            const float px = M[0] * qx, py = M[3] * qx, pz = M[6] * qx;
+            for (int jj = 0; jj < kResampleSS; jj++) {
+                const float ax = fmaf(M[1], qy[jj], px), ay = fmaf(M[4], qy[jj], py), az = fmaf(M[7], qy[jj], pz);
+                float val[kResampleSS];
 #pragma unroll
-            for (int kk = 0; kk < kResampleSS; kk++) {
-                const float by = fmaf(M[5], qz[kk], ay);
-                const float dd = (by - B.lim_min) * B.inv_ls;
-                const float bx = fmaf(M[2], qz[kk], ax);
-                const float bz = fmaf(M[8], qz[kk], az);
-                const float tx = bx * B.kx, tz = bz * B.kz;
-                const float r0 = __builtin_amdgcn_rcpf(by);
-                float rinv = fmaf(r0, fmaf(-by, r0, 1.0f), r0);
-                if (!RCP_EXACT) {
-                    // all-ones significand: the quotient's significand is 0x000001, and the step
-                    // gives it or (from below) the tie 0x000000
-                    const uint32_t t = (__float_as_uint(by) & 0x7FFFFFu) + 1u;  // 0x800000 iff all ones
-                    rinv = __uint_as_float(__float_as_uint(rinv) | (t >> 23));
+                for (int kk = 0; kk < kResampleSS; kk++) {
+                    const float by = fmaf(M[5], qz[kk], ay);
+                    const float dd = (by - B.lim_min) * B.inv_ls;
+                    const float bx = fmaf(M[2], qz[kk], ax);
+                    const float bz = fmaf(M[8], qz[kk], az);
+                    const float tx = bx * B.kx, tz = bz * B.kz;
+                    const float r0 = __builtin_amdgcn_rcpf(by);
+                    float rinv = fmaf(r0, fmaf(-by, r0, 1.0f), r0);
+                    if (!RCP_EXACT) {
+                        // all-ones significand: the quotient's significand is 0x000001, and the step
+                        // gives it or (from below) the tie 0x000000
+                        const uint32_t t = (__float_as_uint(by) & 0x7FFFFFu) + 1u;  // 0x800000 iff all ones
+                        rinv = __uint_as_float(__float_as_uint(rinv) | (t >> 23));
+                    }
+                    const float u = fmaf(tx, rinv, half);
+                    const float w = fmaf(tz, rinv, half);
+                    const int fd = floor_int(dd), fu = floor_int(u), fw = floor_int(w);
+                    const bool ok = ((uint32_t)fd < Lu) & ((uint32_t)(fu | fw) < (uint32_t)kFmap);
+                    const uint32_t boff =
+                        ok ? (((uint32_t)fd * kNpix + (uint32_t)fw * kFmap + (uint32_t)fu) << 2) : kOutside;
+                    val[kk] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)boff, 0, 0));
                 }
-                const float u = fmaf(tx, rinv, half);
-                const float w = fmaf(tz, rinv, half);
-                const int fd = floor_int(dd), fu = floor_int(u), fw = floor_int(w);
-                const bool ok = ((uint32_t)fd < Lu) & ((uint32_t)(fu | fw) < (uint32_t)kFmap);
-                const uint32_t boff = ok ? (((uint32_t)fd * kNpix + (uint32_t)fw * kFmap + (uint32_t)fu) << 2) : kOutside;
-                val[kk] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)boff, 0, 0));
-            }
 #pragma unroll
-            for (int kk = 0; kk < kResampleSS; kk++) acc += val[kk];
+                for (int kk = 0; kk < kResampleSS; kk++) acc += val[kk];
+            }
+        }
+        // the round's partial sums, in plane order, onto the group's first lane (the lanes of a
+        // group share the voxel, so they all reach this point together)
+        sum += acc;
+        for (int d = 1; d < k; d++) {
+            const float o = __shfl(acc, lane + d, 64);
+            if (r * k + d < kResampleSS) sum += o;
         }
     }
-    return acc * (1.0f / (kResampleSS * kResampleSS * kResampleSS));
+    return sub == 0 ? sum * (1.0f / (kResampleSS * kResampleSS * kResampleSS)) : 0.0f;
 }
 
 // At most 64 VGPRs (8 wavefronts per SIMD): the sample gathers need the latency hiding; with the
 // compiler's own choice (126 VGPRs, 4 wavefronts) the kernel runs 25 % slower.
+// A workgroup takes one (z-plane, beamlet) segment: the rows [y_lo, y_hi) the count pass found to
+// hold candidates, their x-ranges and prefix, then the candidates 256 at a time, one lane each; a
+// wavefront left with at most 32 candidates (the segment's last) gives each voxel 2, 3 or 5 lanes,
+// which split its x-sample planes (voxel_value).
 template <bool RCP_EXACT>
 __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void resample_values_kernel(
     const BeamletDev* __restrict__ beams, int X, int Y, int Z, float vsx, float vsy, float vsz, float extent,
-    const float* __restrict__ dose, const uint64_t* __restrict__ cand_off, uint64_t cap,
-    uint32_t* __restrict__ overflow, uint64_t* __restrict__ tmp_cols, float* __restrict__ tmp_vals,
+    const float* __restrict__ dose, const uint64_t* __restrict__ cand_off, const uint2* __restrict__ cand_rows,
+    uint64_t cap, uint32_t* __restrict__ overflow, uint64_t* __restrict__ tmp_cols, float* __restrict__ tmp_vals,
     uint32_t* __restrict__ nnz_count, uint32_t* __restrict__ rowmax_bits) {
-    extern __shared__ uint32_t s_rows[];  // x0[Y], off[Y + 1]
+    extern __shared__ uint32_t s_rows[];  // x0[Y], off[Y + 1] of rows y_lo + i
     __shared__ uint32_t s_wave[kRowThreads / 64 + 1];
     uint32_t* s_x0 = s_rows;
     uint32_t* s_off = s_rows + Y;
@@ -283,31 +321,33 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(8, 
     }
     const BeamletDev& B = beams[b];
     const double rho = rho_of(vsx, vsy, vsz);
+    const uint2 yr = cand_rows[seg];
+    const int y_lo = (int)yr.x, n_rows = (int)(yr.y - yr.x);
 
     // row ranges and their prefix (rows handled in chunks of blockDim)
     uint32_t carry = 0;
-    for (int y0 = 0; y0 < Y; y0 += blockDim.x) {
-        const int y = y0 + threadIdx.x;
+    for (int r0 = 0; r0 < n_rows; r0 += blockDim.x) {
+        const int r = r0 + threadIdx.x;
         uint32_t len = 0;
-        if (y < Y) {
+        if (r < n_rows) {
             int x0, x1;
-            row_interval(B, y, z, X, vsx, vsy, vsz, rho, x0, x1);
-            s_x0[y] = (uint32_t)x0;
+            row_interval(B, y_lo + r, z, X, vsx, vsy, vsz, rho, x0, x1);
+            s_x0[r] = (uint32_t)x0;
             len = (uint32_t)(x1 - x0);
         }
         uint32_t tot;
         const uint32_t ex = block_exclusive_scan(len, s_wave, tot);
-        if (y < Y) s_off[y] = carry + ex;
+        if (r < n_rows) s_off[r] = carry + ex;
         carry += tot;
     }
-    if (threadIdx.x == 0) s_off[Y] = carry;
+    if (threadIdx.x == 0) s_off[n_rows] = carry;
     __syncthreads();
     const uint32_t total = carry;
-    const uint64_t base = cand_off[(size_t)b * Z + z];
+    const uint64_t base = cand_off[seg];
     if (base + total > cap) {  // staging too small: report, write nothing (the caller re-runs)
         if (threadIdx.x == 0) {
             overflow[0] = 1u;
-            nnz_count[(size_t)b * Z + z] = 0u;
+            nnz_count[seg] = 0u;
         }
         return;
     }
@@ -315,25 +355,32 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(8, 
     const float half_diag = sqrtf(vsx * vsx + vsy * vsy + vsz * vsz) * 0.5f;
     const float svx = 8 * vsx, svy = 8 * vsy, svz = 8 * vsz;
     const float half_sdiag = sqrtf(svx * svx + svy * svy + svz * svz) * 0.5f;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
 
     uint32_t written = 0;
     float vmax = 0.0f;
     for (uint32_t c0 = 0; c0 < total; c0 += blockDim.x) {
-        const uint32_t c = c0 + threadIdx.x;
+        // this wavefront's candidates of the pass, and the lanes each of them gets
+        const uint32_t wbase = c0 + 64u * wave;
+        const uint32_t m = total > wbase ? min(total - wbase, 64u) : 0u;
+        const int k = m > 32u ? 1 : (m > 21u ? 2 : (m > 12u ? 3 : 5));
+        const uint32_t grp = lane / (uint32_t)k, sub = lane - grp * (uint32_t)k;
+        const bool mine = grp < m;
+        const uint32_t c = wbase + grp;
         float v = 0.0f;
         int x = 0, y = 0;
-        if (c < total) {
-            // row containing candidate c: largest y with s_off[y] <= c
-            int lo = 0, hi = Y - 1;
+        if (mine) {
+            // row containing candidate c: largest r with s_off[r] <= c
+            int lo = 0, hi = n_rows - 1;
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
                 if (s_off[mid] <= c) lo = mid; else hi = mid - 1;
             }
-            y = lo;
-            x = (int)(s_x0[y] + (c - s_off[y]));
-            v = voxel_value<RCP_EXACT>(B, dose, x, y, z, vsx, vsy, vsz, extent, half_diag, half_sdiag);
+            y = y_lo + lo;
+            x = (int)(s_x0[lo] + (c - s_off[lo]));
+            v = voxel_value<RCP_EXACT>(B, dose, x, y, z, vsx, vsy, vsz, extent, half_diag, half_sdiag, k, (int)sub);
         }
-        const uint32_t nz = (c < total && v != 0.0f) ? 1u : 0u;
+        const uint32_t nz = (mine && sub == 0u && v != 0.0f) ? 1u : 0u;
         vmax = fmaxf(vmax, v);
         uint32_t tot;
         const uint32_t pos = block_exclusive_scan(nz, s_wave, tot);
@@ -344,7 +391,7 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(8, 
         }
         written += tot;
     }
-    if (threadIdx.x == 0) nnz_count[(size_t)b * Z + z] = written;
+    if (threadIdx.x == 0) nnz_count[seg] = written;
     if (rowmax_bits) {
         // beamlet maximum for the relative threshold (values are >= 0, so their bit patterns order
         // like the floats)
@@ -438,29 +485,30 @@ __global__ void row_offsets_kernel(int n, int Z, const uint64_t* __restrict__ nn
 }  // namespace
 
 hipError_t launch_resample_count(const BeamletDev* beams, int n, const fd_volume& vol, uint32_t* cand_count,
-                                 hipStream_t stream) {
+                                 uint2* cand_rows, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     dim3 grid((vol.dim[2] + kCountPlanes - 1) / kCountPlanes, n), block(kRowThreads);
     hipLaunchKernelGGL(resample_count_kernel, grid, block, 0, stream, beams, (int)vol.dim[0], (int)vol.dim[1],
-                       (int)vol.dim[2], vol.voxel_size[0], vol.voxel_size[1], vol.voxel_size[2], cand_count);
+                       (int)vol.dim[2], vol.voxel_size[0], vol.voxel_size[1], vol.voxel_size[2], cand_count,
+                       cand_rows);
     return hipGetLastError();
 }
 
 hipError_t launch_resample_values(const BeamletDev* beams, int n, const fd_volume& vol, float extent,
-                                  const float* dose_bev, const uint64_t* cand_off, uint64_t cap, uint32_t* overflow,
-                                  uint64_t* tmp_cols, float* tmp_vals, uint32_t* nnz_count, uint32_t* rowmax_bits,
-                                  bool rcp_exact, hipStream_t stream) {
+                                  const float* dose_bev, const uint64_t* cand_off, const uint2* cand_rows,
+                                  uint64_t cap, uint32_t* overflow, uint64_t* tmp_cols, float* tmp_vals,
+                                  uint32_t* nnz_count, uint32_t* rowmax_bits, bool rcp_exact, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     dim3 grid(vol.dim[2], n), block(kRowThreads);
     const size_t shm = (2 * (size_t)vol.dim[1] + 1) * sizeof(uint32_t);
     if (rcp_exact)
         hipLaunchKernelGGL(resample_values_kernel<true>, grid, block, shm, stream, beams, (int)vol.dim[0],
                            (int)vol.dim[1], (int)vol.dim[2], vol.voxel_size[0], vol.voxel_size[1], vol.voxel_size[2],
-                           extent, dose_bev, cand_off, cap, overflow, tmp_cols, tmp_vals, nnz_count, rowmax_bits);
+                           extent, dose_bev, cand_off, cand_rows, cap, overflow, tmp_cols, tmp_vals, nnz_count, rowmax_bits);
     else
         hipLaunchKernelGGL(resample_values_kernel<false>, grid, block, shm, stream, beams, (int)vol.dim[0],
                            (int)vol.dim[1], (int)vol.dim[2], vol.voxel_size[0], vol.voxel_size[1], vol.voxel_size[2],
-                           extent, dose_bev, cand_off, cap, overflow, tmp_cols, tmp_vals, nnz_count, rowmax_bits);
+                           extent, dose_bev, cand_off, cand_rows, cap, overflow, tmp_cols, tmp_vals, nnz_count, rowmax_bits);
     return hipGetLastError();
 }
 
