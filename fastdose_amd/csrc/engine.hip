@@ -429,8 +429,10 @@ constexpr int kTabMinShare = 4;
 // w.tab_stage and reserves the streams and headers.  Capacities: within one slice an azimuth
 // crosses at most ceil(|dx / dz|) + 1 x and ceil(|dy / dz|) + 1 y cell boundaries plus the slice
 // end (dx, dy, dz the lattice direction); twice that plus 4 bounds its steps per slice, a chunk
-// takes at most cccs_chunk_steps() of them, and a walk covers at most Lmax slices.  Sets
-// w.tab_n_gt (0: no tables).
+// takes at most cccs_chunk_steps() of them, and a walk covers at most Lmax slices.  A chunk's
+// records never straddle a block of cccs_chunk_steps() records: the space skipped at a block end
+// is smaller than the chunk placed after it, so a stream needs at most twice its records (plus a
+// block).  Sets w.tab_n_gt (0: no tables).
 int plan_tables(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams, int n) {
     w.tab_n_gt = 0;
     const int n_phi = ctx->cones.n_phi, n_fwd = ctx->n_fwd;
@@ -498,7 +500,7 @@ int plan_tables(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* bea
                 steps = std::max(steps, 4.0 + 2.0 * std::ceil(ax / az) + 2.0 * std::ceil(ay / az));
             }
             const double L1 = (double)geoms[g].Lmax + 1.0;
-            const double rec_cap = L1 * steps + chunk, chunk_cap = L1 * std::ceil(steps / chunk) + 2.0;
+            const double rec_cap = 2.0 * L1 * steps + chunk, chunk_cap = L1 * std::ceil(steps / chunk) + 2.0;
             if (!(rec_cap < 4e9 && chunk_cap < 4e9)) {
                 set_error("CCCS march table beyond 2^32 records (cone too close to 90 degrees)");
                 return FD_ERR_CAPACITY;

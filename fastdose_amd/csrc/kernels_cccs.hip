@@ -527,7 +527,8 @@ static_assert(sizeof(StepRec) == 32, "StepRec is two 16-byte LDS accesses");
 //    voxel row whose rays crossed the y edge, bit 28: a step of more than one cell (test each ray),
 //    bit 29: the previous step's column reset is done by this step's x shift (zero fill), bits 30 /
 //    31: the x / y shift is one voxel.
-constexpr int kRecCap = 32;       // steps per azimuth per chunk
+constexpr int kRecCap = 32;       // steps per azimuth per chunk (and records per march table block)
+static_assert((kRecCap & (kRecCap - 1)) == 0, "table blocks are aligned by masking");
 constexpr int kHdr = kMaxPhi + 2; // per chunk: step count of each azimuth, slice done, next slice
 constexpr uint32_t kColReset = 1u << 18, kRowReset = 1u << 23, kGeneralReset = 1u << 28, kZeroFillX = 1u << 29;
 // bit 30 / 31: the shift along x / y is exactly one voxel (its direction is the field's sign bit, 4 / 9)
@@ -579,10 +580,10 @@ __device__ __forceinline__ CcLds carve_lds(float4* base, int n_phi) {
     return S;
 }
 
-// LDS of cccs_tab_kernel: two chunk buffers, each one record area per compute wavefront (kTabRec
-// float4: the chunk's records and one more, the read past a chunk's last) and the chunk header,
-// then the slabs, rho, cumulative length and per-azimuth doses as in carve_lds.  Chunk c's
-// buffer is filled by loads into LDS issued during chunk c - 1.
+// LDS of cccs_tab_kernel: two block buffers per compute wavefront (kTabRec float4: a block of
+// kRecCap records of its azimuth's stream and one more, the read past a chunk's last), two chunk
+// headers, then the slabs, rho, cumulative length and per-azimuth doses as in carve_lds.  Block
+// b + 1 is loaded into LDS while the chunks of block b run; chunk c's header during chunk c - 1.
 constexpr int kTabRec = 2 * (kRecCap + 1);
 __host__ __device__ constexpr size_t tab_lds_bytes(int n_phi) {
     return (size_t)2 * n_phi * kTabRec * sizeof(float4) + (size_t)2 * kTabHdr * sizeof(int) +
@@ -636,8 +637,8 @@ __device__ __forceinline__ void slab_regs(const CcBlock& K, int z, float& pf0, f
 }
 // End of a chunk, executed by every wavefront: store the prefetched slab, barrier, and after a
 // slice end the combine of the slice into P.  Returns false once the beamlet is finished.
-// DMA: this wavefront has loads into LDS in flight (cccs_tab_kernel's next chunk), complete before
-// the barrier (vector memory operations complete in issue order).
+// DMA: this wavefront may have loads into LDS in flight (cccs_tab_kernel's next chunk header or
+// record block), complete before the barrier (vector memory operations complete in issue order).
 __device__ __forceinline__ bool chunk_end(const CcBlock& K, const CcLds& S, int done, int sn, float pf0,
                                           float pf1, int& s, int& si, int& written, bool dma = false) {
     if (done && sn < K.L && K.prefetch) {
@@ -704,7 +705,7 @@ struct MarchConst {
 struct TabOut {
     float4* rec;       // azimuth 0's stream
     uint32_t cap;      // records per azimuth
-    const int* ofs;    // LDS: records of each azimuth written before this chunk
+    int* ofs;          // LDS: records of each azimuth written before this chunk; set to the chunk's start
 };
 
 // Returns false (TABLE only) when a table stream cannot take the chunk; nothing is written then.
@@ -751,7 +752,20 @@ __device__ __forceinline__ bool march_chunk(MarchLane& M, const MarchConst& C, i
         hdr[kMaxPhi] = done ? 1 : 0;
         hdr[kMaxPhi + 1] = max((int)rintf(M.bzp), s_cur + 1);  // the phi-0 march decides the next slice
     }
-    if (TABLE && __ballot(lane < n_phi && (uint32_t)T.ofs[lane] + (uint32_t)cnt > T.cap) != 0) return false;
+    if (TABLE) {
+        // a chunk's records never straddle a block of kRecCap records of the stream: one that does
+        // not fit in the rest of the current block starts the next one (the compute wavefronts load
+        // and finish the stream block by block)
+        bool over = false;
+        if (lane < n_phi) {
+            uint32_t o = (uint32_t)T.ofs[lane];
+            if ((o & (kRecCap - 1)) + (uint32_t)cnt > (uint32_t)kRecCap)
+                o = (o + kRecCap - 1) & ~(uint32_t)(kRecCap - 1);
+            over = o + (uint32_t)cnt > T.cap;
+            T.ofs[lane] = (int)o;
+        }
+        if (__ballot(over) != 0) return false;
+    }
     // phase 2: records; k is the largest step count of the chunk
     const bool mine = C.lf < n_phi;
     const int f = mine ? C.lf : 0;
@@ -991,18 +1005,21 @@ __device__ __forceinline__ void load_to_lds16(const void* base, uint32_t off, vo
     asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(off), "s"(base), "s"(a) : "m0");
 }
 
-// Chunk `chunk` of a march table into LDS: by wavefront 0, the chunk's header (4 lanes x 16 bytes)
-// into `hdr`, and the kRecCap records of this azimuth's stream from record `ofs` on (64 lanes x 16
-// bytes; indices clamped into the stream, what lies past the chunk's records is never read) into
-// `rec`.
-__device__ __forceinline__ void tab_load_chunk(const TabRead& TR, uint32_t chunk, uint32_t ofs, float4* rec, int* hdr,
-                                               int wave) {
+// Header of chunk `chunk` of a march table into LDS at `hdr`, by wavefront 0 (4 lanes x 16 bytes).
+__device__ __forceinline__ void tab_load_header(const TabRead& TR, uint32_t chunk, int* hdr, int wave) {
     const uint32_t lane = threadIdx.x & 63;
     if (wave == 0 && lane < 4)
         load_to_lds16(TR.hdr + (uint64_t)min(chunk, TR.chunks - 1) * kTabHdr, 16 * lane, hdr);
-    const uint32_t i = min(2 * ofs + lane, 2 * TR.cap - 1);
+}
+// Block `blk` of this azimuth's stream (records kRecCap blk ... kRecCap (blk + 1) - 1; 64 lanes x
+// 16 bytes, indices clamped into the stream: what lies past its last record is never read) into
+// LDS at `rec`.
+__device__ __forceinline__ void tab_load_block(const TabRead& TR, uint32_t blk, float4* rec) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t i = min(2 * kRecCap * blk + lane, 2 * TR.cap - 1);
     load_to_lds16(TR.rec, 16 * i, rec);
 }
+static_assert(2 * kRecCap == kWave, "a table block is one 16-byte load per lane");
 
 // Compute wavefront of azimuth `wave`: lane l owns voxels (x = l & 15, y = 4 (l >> 4) + r).  The
 // X states are kept pre-scaled by A/a and B/b, and the dose and cumulative length are summed in
@@ -1035,44 +1052,57 @@ __device__ __forceinline__ void cccs_compute(const CcBlock& K, const CcLds& S, i
     for (int r = 0; r < kSlots; r++) fiy[r] = (float)(4 * g + r);
 
     if (TAB) {
-        tab_load_chunk(TR, 0, 0, S.rec + wave * kTabRec, S.hdr, wave);
+        tab_load_header(TR, 0, S.hdr, wave);
+        tab_load_block(TR, 0, S.rec + wave * kTabRec);
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     }
-    __syncthreads();  // chunk 0 and slab 0 are in LDS
+    __syncthreads();  // chunk 0's header, block 0 and slab 0 are in LDS
     int s = 0, buf = 0, si = 0, written = 0;  // si: ordinal of the slice being processed
-    uint32_t tc = 0, tofs = 0;                 // TAB: chunk, records of my azimuth before it
+    // TAB: chunk, stream position after the previous chunk, block finished last (-1: none)
+    uint32_t tc = 0, tofs = 0;
+    int blk = -1;
     bool fresh = true;
     float TA[kSlots], TB[kSlots], nRa[kSlots], nRb[kSlots], D[kSlots], Cs[kSlots];
     for (;;) {
         int done, sn, cnt;
         const float4* rp;
         if (TAB) {
-            // this chunk's header and my azimuth's records are in LDS (loaded during the previous
-            // chunk); the records are finished in place with the beamlet terms (the arithmetic of the
-            // march wavefront's phase 2, same operation order), then the next chunk's loads start
+            // this chunk's header is in LDS (loaded during the previous chunk).  My azimuth's records
+            // of the chunk lie in one block of the stream (the table's placement rule, repeated
+            // here); a chunk that starts a new block finishes the whole block in place with the
+            // beamlet terms (the arithmetic of the march wavefront's phase 2, same operation order;
+            // the block arrived during an earlier chunk) and starts loading the next block into the
+            // other buffer, which the previous block no longer needs.
             const int* h = S.hdr + (tc & 1) * kTabHdr;
             done = h[kTabHdr - 2];
             sn = h[kTabHdr - 1];
             cnt = min(max(__builtin_amdgcn_readfirstlane(h[wave]), 0), kRecCap);
-            float4* my = S.rec + ((tc & 1) * n_phi + wave) * kTabRec;
-            if (lane < cnt) {
-                const float4 g0 = my[2 * lane], g1 = my[2 * lane + 1];
-                // g0 = (dzs, z1, x0 - 8, y0 - 8), g1 = (bsx dx, bsy dy, w, gx | gy << 16)
-                const float fe = fmaf(g0.y, TR.ls, TR.lim_min) * TR.inv_sad;
-                const float df = g0.x * TR.inv_sad;
-                const float gx = TR.bsx * df, gy = TR.bsy * df;
-                const int gxy = __float_as_int(g1.w);
-                float hc;
-                my[2 * lane] = heron_record(gx, gy, fmaf(gx, g0.z, g1.x * fe), fmaf(gy, g0.w, g1.y * fe),
-                                            g0.x * g0.x, hc);
-                my[2 * lane + 1] = make_float4(hc, g1.z, __int_as_float((int)(short)(gxy & 0xFFFF)),
-                                               __int_as_float(gxy >> 16));
+            uint32_t o = tofs;
+            if ((o & (kRecCap - 1)) + (uint32_t)cnt > (uint32_t)kRecCap)
+                o = (o + kRecCap - 1) & ~(uint32_t)(kRecCap - 1);
+            const int ob = (int)(o / kRecCap);
+            float4* my = S.rec + ((ob & 1) * n_phi + wave) * kTabRec;
+            if (cnt > 0 && ob != blk) {
+                if (lane < kRecCap) {
+                    const float4 g0 = my[2 * lane], g1 = my[2 * lane + 1];
+                    // g0 = (dzs, z1, x0 - 8, y0 - 8), g1 = (bsx dx, bsy dy, w, gx | gy << 16)
+                    const float fe = fmaf(g0.y, TR.ls, TR.lim_min) * TR.inv_sad;
+                    const float df = g0.x * TR.inv_sad;
+                    const float gx = TR.bsx * df, gy = TR.bsy * df;
+                    const int gxy = __float_as_int(g1.w);
+                    float hc;
+                    my[2 * lane] = heron_record(gx, gy, fmaf(gx, g0.z, g1.x * fe), fmaf(gy, g0.w, g1.y * fe),
+                                                g0.x * g0.x, hc);
+                    my[2 * lane + 1] = make_float4(hc, g1.z, __int_as_float((int)(short)(gxy & 0xFFFF)),
+                                                   __int_as_float(gxy >> 16));
+                }
+                blk = ob;
+                tab_load_block(TR, (uint32_t)ob + 1, S.rec + (((ob + 1) & 1) * n_phi + wave) * kTabRec);
             }
-            rp = my;
-            tofs += (uint32_t)cnt;
+            rp = my + 2 * (o & (kRecCap - 1));
+            tofs = o + (uint32_t)cnt;
             tc++;
-            tab_load_chunk(TR, tc, tofs, S.rec + ((tc & 1) * n_phi + wave) * kTabRec, S.hdr + (tc & 1) * kTabHdr,
-                           wave);
+            tab_load_header(TR, tc, S.hdr + (tc & 1) * kTabHdr, wave);
         } else {
             done = S.hdr[buf * kHdr + kMaxPhi];
             sn = S.hdr[buf * kHdr + kMaxPhi + 1];
