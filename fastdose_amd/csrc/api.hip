@@ -748,14 +748,20 @@ int fd_dosemat_add_beams(fd_ctx* ctx, fd_dosemat* m, const fd_beam* beamlets, co
         int k1 = k0;
         uint64_t n = 0;
         while (k1 < n_beams && (k1 == k0 || n + (uint64_t)counts[k1] <= (uint64_t)max_chunk)) n += counts[k1++];
+        // the batch's entries straight into the ensemble's free space when they fit there
+        const uint64_t have = std::min<uint64_t>(m->columns.cap, m->values.cap);
+        const uint64_t room = have > m->used_nnz ? have - m->used_nnz : 0;
+        const CsrDst sink{m->columns.ptr + m->used_nnz, m->values.ptr + m->used_nnz, room};
         FD_TRY(run_batch(ctx, ctx->ws, options, beamlets + first[k0], (int)n, fluence + first[k0] * kNpix, extent,
-                         ctx->threshold, s, &res));
+                         ctx->threshold, s, &res, room ? &sink : nullptr));
         // one matrix per beam: offsets re-based to each beam's first row
         host_off.clear();
         host_off.reserve(n + (k1 - k0));
-        FD_TRY(grow_keep(m->columns, m->used_nnz, m->used_nnz + std::max<uint64_t>(res.nnz, 1), s));
-        FD_TRY(grow_keep(m->values, m->used_nnz, m->used_nnz + std::max<uint64_t>(res.nnz, 1), s));
-        if (res.nnz) {
+        if (!res.in_sink) {
+            FD_TRY(grow_keep(m->columns, m->used_nnz, m->used_nnz + std::max<uint64_t>(res.nnz, 1), s));
+            FD_TRY(grow_keep(m->values, m->used_nnz, m->used_nnz + std::max<uint64_t>(res.nnz, 1), s));
+        }
+        if (res.nnz && !res.in_sink) {
             FD_HIP(hipMemcpyAsync(m->columns.ptr + m->used_nnz, ctx->ws.out_cols.ptr, res.nnz * sizeof(uint64_t),
                                   hipMemcpyDeviceToDevice, s));
             FD_HIP(hipMemcpyAsync(m->values.ptr + m->used_nnz, ctx->ws.out_vals.ptr, res.nnz * sizeof(float),

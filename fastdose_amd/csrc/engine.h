@@ -160,8 +160,9 @@ struct Workspace {
 
 // CSR rows of one finished batch, as the host sees them.
 struct BatchResult {
-    std::vector<uint64_t> rowoff;  // n + 1 offsets into ws.out_cols / ws.out_vals
+    std::vector<uint64_t> rowoff;  // n + 1 offsets into the entries' arrays
     uint64_t nnz = 0;
+    bool in_sink = false;          // the entries went to the caller's sink (else ws.out_cols / ws.out_vals)
 };
 
 }  // namespace fdi
@@ -226,14 +227,15 @@ int run_cccs(const fd_ctx* ctx, Workspace& w, int options, int n, int max_L, con
 // staging overflows).  thr > 0 keeps only entries >= thr * (the beamlet's maximum value).
 int run_resample_csr(const fd_ctx* ctx, Workspace& w, const fd_beam* beams, int n, const float* dose, float extent,
                      float thr, const fd_csr& out, uint64_t* nnz_total, hipStream_t stream);
-int run_resample_compact(const fd_ctx* ctx, Workspace& w, int n, float thr, uint64_t* offsets, uint64_t* columns,
-                         float* values, uint64_t capacity, hipStream_t stream);
+int run_resample_compact(const fd_ctx* ctx, Workspace& w, int n, float thr, uint64_t* offsets, const CsrDst& dst0,
+                         const CsrDst& dst1, hipStream_t stream);
 
 // The whole path for a batch of beamlets (host descriptors and host fluence, 256 floats each):
-// Terma -> CCCS -> resampling, CSR rows into w.out_cols / w.out_vals, one host synchronisation
-// (two when the resampler staging estimate was too small).
+// Terma -> CCCS -> resampling, one host synchronisation (two when the resampler staging estimate
+// was too small).  The CSR columns and values go to *sink when its capacity holds them (the
+// device decides; out->in_sink tells), else to w.out_cols / w.out_vals.
 int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams, int n, const float* fluence,
-              float extent, float thr, hipStream_t stream, BatchResult* out);
+              float extent, float thr, hipStream_t stream, BatchResult* out, const CsrDst* sink = nullptr);
 
 // Device bytes a batch of these beamlets needs (BEV volumes, cone partials, resampler staging).
 size_t batch_footprint(const fd_ctx* ctx, const fd_beam* beams, int n);

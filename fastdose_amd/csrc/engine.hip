@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 #include "engine.h"
 #include "geometry.h"
@@ -138,6 +139,26 @@ void build_descriptors(const fd_beam* beams, int n, uint64_t stride, BeamletDev*
     }
 }
 
+// memcpy of a large host buffer in up to 8 parts on threads of their own (one core copies into
+// pinned memory at a few GB/s: 16 MiB of fluence took about 2 ms on one)
+void copy_parallel(void* dst, const void* src, size_t bytes) {
+    constexpr size_t kPart = size_t(4) << 20;
+    const size_t parts = std::min<size_t>(8, bytes / kPart);
+    if (parts < 2) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t step = (bytes / parts + 63) & ~size_t(63);
+    std::thread th[8];
+    for (size_t p = 1; p < parts; p++) {
+        const size_t a = p * step, b = std::min(bytes, a + step);
+        if (a < b) th[p] = std::thread([=] { std::memcpy((char*)dst + a, (const char*)src + a, b - a); });
+    }
+    std::memcpy(dst, src, std::min(bytes, step));
+    for (size_t p = 1; p < parts; p++)
+        if (th[p].joinable()) th[p].join();
+}
+
 // Pinned staging of the descriptors (beams non-null) and optionally the fluence of a batch; waits
 // until the previous upload from this workspace has left the staging area.
 int stage_inputs(Workspace& w, const fd_beam* beams, int n, uint64_t stride, const float* fluence_host) {
@@ -159,7 +180,7 @@ int stage_inputs(Workspace& w, const fd_beam* beams, int n, uint64_t stride, con
             FD_HIP(hipHostMalloc((void**)&w.flu_stage, nf * sizeof(float), hipHostMallocDefault));
             w.flu_cap = nf;
         }
-        std::memcpy(w.flu_stage, fluence_host, nf * sizeof(float));
+        copy_parallel(w.flu_stage, fluence_host, nf * sizeof(float));
     }
     return FD_OK;
 }
@@ -383,7 +404,8 @@ int run_resample_csr(const fd_ctx* ctx, Workspace& w, const fd_beam* beams, int 
         FD_TRY(w.tmp_cols.reserve(cap));
         FD_TRY(w.tmp_vals.reserve(cap));
         FD_TRY(launch_values(ctx, w, n, dose, extent, thr, cap, stream));
-        FD_TRY(run_resample_compact(ctx, w, n, thr, out.offsets, out.columns, out.values, capacity, stream));
+        FD_TRY(run_resample_compact(ctx, w, n, thr, out.offsets, CsrDst{out.columns, out.values, capacity}, CsrDst{},
+                                    stream));
         FD_TRY(fetch_totals(ctx, w, n, stream, false));
         FD_HIP(hipStreamSynchronize(stream));
         return FD_OK;
@@ -404,12 +426,12 @@ int run_resample_csr(const fd_ctx* ctx, Workspace& w, const fd_beam* beams, int 
     return FD_OK;
 }
 
-int run_resample_compact(const fd_ctx* ctx, Workspace& w, int n, float thr, uint64_t* offsets, uint64_t* columns,
-                         float* values, uint64_t capacity, hipStream_t stream) {
+int run_resample_compact(const fd_ctx* ctx, Workspace& w, int n, float thr, uint64_t* offsets, const CsrDst& dst0,
+                         const CsrDst& dst1, hipStream_t stream) {
     w.mark(FD_T_RESAMPLE_COMPACT, 0, stream);
     hipError_t e = launch_resample_compact(n, (int)ctx->vol.dim[2], w.cand_off.ptr, w.nnz_off.ptr, w.nnz_count.ptr,
                                            w.tmp_cols.ptr, w.tmp_vals.ptr, thr > 0.0f ? w.rowmax.ptr : nullptr, thr,
-                                           offsets, columns, values, capacity, stream);
+                                           offsets, dst0, dst1, stream);
     w.mark(FD_T_RESAMPLE_COMPACT, 1, stream);
     if (e != hipSuccess) {
         set_error(std::string("resample compact: ") + hipGetErrorString(e));
@@ -529,7 +551,7 @@ int plan_tables(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* bea
 }  // namespace
 
 int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams, int n, const float* fluence,
-              float extent, float thr, hipStream_t stream, BatchResult* out) {
+              float extent, float thr, hipStream_t stream, BatchResult* out, const CsrDst* sink) {
     out->rowoff.assign(1, 0);
     out->nnz = 0;
     if (n == 0) return FD_OK;
@@ -558,6 +580,9 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
     FD_TRY(plan_tables(ctx, w, options, beams, n));
     FD_TRY(stage_inputs(w, beams, n, stride, fluence));
     const CccsTables tabs{w.tab_ent.ptr, w.tab_rec.ptr, w.tab_hdr.ptr, w.flags.ptr + 2, w.tab_n_gt};
+    // the batch's CSR entries: into the caller's sink when they fit there, else the workspace
+    CsrDst dst1{w.out_cols.ptr, w.out_vals.ptr, cap};
+    CsrDst dst0 = sink ? *sink : dst1;
     if (w.tab_n_gt)
         for (int i = 0; i < n; i++) w.stage[i].geom = w.tab_geom[i];
     w.host_totals[3] = 0;
@@ -576,7 +601,7 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
                         w.tab_n_gt ? &tabs : nullptr));
         FD_TRY(launch_count(ctx, w, n, st));
         FD_TRY(launch_values(ctx, w, n, w.dose_bev.ptr, extent, thr, cap, st));
-        FD_TRY(run_resample_compact(ctx, w, n, thr, w.rowoff.ptr, w.out_cols.ptr, w.out_vals.ptr, cap, st));
+        FD_TRY(run_resample_compact(ctx, w, n, thr, w.rowoff.ptr, dst0, dst1, st));
         FD_TRY(fetch_totals(ctx, w, n, st, true));
         if (w.tab_n_gt)  // the table overflow flag into the low half of host_totals[3]
             FD_HIP(hipMemcpyAsync(w.host_totals + 3, w.flags.ptr + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -633,7 +658,9 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
         FD_TRY(w.out_cols.reserve(cap));
         FD_TRY(w.out_vals.reserve(cap));
         FD_TRY(launch_values(ctx, w, n, w.dose_bev.ptr, extent, thr, cap, stream));
-        FD_TRY(run_resample_compact(ctx, w, n, thr, w.rowoff.ptr, w.out_cols.ptr, w.out_vals.ptr, cap, stream));
+        dst1 = CsrDst{w.out_cols.ptr, w.out_vals.ptr, cap};
+        if (!sink) dst0 = dst1;
+        FD_TRY(run_resample_compact(ctx, w, n, thr, w.rowoff.ptr, dst0, dst1, stream));
         FD_TRY(fetch_totals(ctx, w, n, stream, true));
         FD_HIP(hipStreamSynchronize(stream));
         if (w.host_totals[2]) {
@@ -644,6 +671,7 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
     update_candidate_estimate(w, ratio);
     w.fold_all();
     out->nnz = w.host_totals[1];
+    out->in_sink = sink && out->nnz <= dst0.capacity && dst0.columns && dst0.values;
     out->rowoff.assign(w.host_rowoff, w.host_rowoff + n + 1);
     return FD_OK;
 }

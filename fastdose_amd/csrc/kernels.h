@@ -7,6 +7,13 @@
 
 namespace fdi {
 
+// Destination of compacted CSR entries: columns, values and the entries they hold.
+struct CsrDst {
+    uint64_t* columns = nullptr;
+    float* values = nullptr;
+    uint64_t capacity = 0;
+};
+
 // Beamlet / PTV intersection flags of n bundles (d_beamletFlagInit, IMRTBeamBundle.cu:130-316);
 // flags[beams[b].out_off + pixel], max_pixels = the largest fluence_dim^2.
 hipError_t launch_beamlet_flags(const FlagBeamDev* beams, int n, int max_pixels, const uint8_t* mask,
@@ -76,12 +83,14 @@ hipError_t launch_resample_threshold_count(int n, int Z, const uint64_t* cand_of
                                            const float* tmp_vals, const uint32_t* rowmax_bits, float thr,
                                            uint32_t* keep_count, hipStream_t stream);
 // Stage 3: copy the compacted nonzeros into the CSR arrays (with rowmax_bits and thr > 0 only
-// those >= thr * rowmax); row offsets from nnz_off.  When the total (nnz_off[n Z]) exceeds
-// `capacity` entries, the columns and values are left untouched (the offsets are still written).
+// those >= thr * rowmax); row offsets from nnz_off.  The entries go to dst0 when its capacity
+// holds the total (nnz_off[n Z]), else to dst1 when that one does, else nowhere (the columns and
+// values are left untouched; the offsets are still written).  A destination with a null array
+// counts as capacity 0.
 hipError_t launch_resample_compact(int n, int Z, const uint64_t* cand_off, const uint64_t* nnz_off,
                                    const uint32_t* nnz_count, const uint64_t* tmp_cols, const float* tmp_vals,
-                                   const uint32_t* rowmax_bits, float thr, uint64_t* offsets, uint64_t* columns,
-                                   float* values, uint64_t capacity, hipStream_t stream);
+                                   const uint32_t* rowmax_bits, float thr, uint64_t* offsets, CsrDst dst0,
+                                   CsrDst dst1, hipStream_t stream);
 
 // Exclusive scan of count[0..N) (u32) into off[0..N) (u64) plus total[0] = sum.  With scratch
 // (scan_scratch_size(N) u64, nullable) large inputs take a three-pass multi-workgroup scan.

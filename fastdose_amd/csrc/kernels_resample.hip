@@ -419,14 +419,33 @@ __global__ __launch_bounds__(256) void resample_threshold_count_kernel(int Z, co
     if (threadIdx.x == 0) keep_count[seg] = total;
 }
 
+// The arrays the compacted entries go to: the first destination that holds all `total` of them;
+// false (write nothing) when neither does.  The choice is the same in every workgroup.
+__device__ __forceinline__ bool pick_dst(uint64_t total, const CsrDst& d0, const CsrDst& d1, uint64_t*& columns,
+                                         float*& values) {
+    if (total <= d0.capacity) {
+        columns = d0.columns;
+        values = d0.values;
+        return true;
+    }
+    if (total <= d1.capacity) {
+        columns = d1.columns;
+        values = d1.values;
+        return true;
+    }
+    return false;
+}
+
 // Ordered copy of the entries that pass the threshold into the CSR arrays.
 __global__ __launch_bounds__(256) void resample_compact_threshold_kernel(
     int Z, const uint64_t* __restrict__ cand_off, const uint64_t* __restrict__ nnz_off,
     const uint32_t* __restrict__ nnz_count, const uint64_t* __restrict__ tmp_cols, const float* __restrict__ tmp_vals,
-    const uint32_t* __restrict__ rowmax_bits, float thr, const uint64_t* __restrict__ total, uint64_t capacity,
-    uint64_t* __restrict__ columns, float* __restrict__ values) {
+    const uint32_t* __restrict__ rowmax_bits, float thr, const uint64_t* __restrict__ total, CsrDst dst0,
+    CsrDst dst1) {
     __shared__ uint32_t s_wave[256 / 64 + 1];
-    if (total[0] > capacity) return;  // the caller's arrays are too small: write nothing
+    uint64_t* columns;
+    float* values;
+    if (!pick_dst(total[0], dst0, dst1, columns, values)) return;
     const size_t seg = (size_t)blockIdx.y * Z + blockIdx.x;
     const float cut = thr * __uint_as_float(rowmax_bits[blockIdx.y]);
     const uint32_t cnt = nnz_count[seg];
@@ -451,10 +470,11 @@ __global__ __launch_bounds__(256) void resample_compact_kernel(int Z, const uint
                                                                const uint32_t* __restrict__ nnz_count,
                                                                const uint64_t* __restrict__ tmp_cols,
                                                                const float* __restrict__ tmp_vals,
-                                                               const uint64_t* __restrict__ total,
-                                                               uint64_t capacity, uint64_t* __restrict__ columns,
-                                                               float* __restrict__ values) {
-    if (total[0] > capacity) return;  // the caller's arrays are too small: write nothing
+                                                               const uint64_t* __restrict__ total, CsrDst dst0,
+                                                               CsrDst dst1) {
+    uint64_t* columns;
+    float* values;
+    if (!pick_dst(total[0], dst0, dst1, columns, values)) return;
     const size_t seg = (size_t)blockIdx.y * Z + blockIdx.x;
     const uint32_t cnt = nnz_count[seg];
     const uint64_t src = cand_off[seg], dst = nnz_off[seg];
@@ -528,17 +548,19 @@ hipError_t launch_resample_threshold_count(int n, int Z, const uint64_t* cand_of
 
 hipError_t launch_resample_compact(int n, int Z, const uint64_t* cand_off, const uint64_t* nnz_off,
                                    const uint32_t* nnz_count, const uint64_t* tmp_cols, const float* tmp_vals,
-                                   const uint32_t* rowmax_bits, float thr, uint64_t* offsets, uint64_t* columns,
-                                   float* values, uint64_t capacity, hipStream_t stream) {
+                                   const uint32_t* rowmax_bits, float thr, uint64_t* offsets, CsrDst dst0,
+                                   CsrDst dst1, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     dim3 grid(Z, n), block(256);
     const uint64_t* total = nnz_off + (size_t)n * Z;
+    if (!dst0.columns || !dst0.values) dst0.capacity = 0;
+    if (!dst1.columns || !dst1.values) dst1.capacity = 0;
     if (rowmax_bits && thr > 0.0f)
         hipLaunchKernelGGL(resample_compact_threshold_kernel, grid, block, 0, stream, Z, cand_off, nnz_off, nnz_count,
-                           tmp_cols, tmp_vals, rowmax_bits, thr, total, capacity, columns, values);
+                           tmp_cols, tmp_vals, rowmax_bits, thr, total, dst0, dst1);
     else
         hipLaunchKernelGGL(resample_compact_kernel, grid, block, 0, stream, Z, cand_off, nnz_off, nnz_count, tmp_cols,
-                           tmp_vals, total, capacity, columns, values);
+                           tmp_vals, total, dst0, dst1);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // nnz_off has n*Z entries followed by the total at index n*Z
