@@ -63,16 +63,24 @@ __global__ __launch_bounds__(256) void expand_density_kernel(const float* __rest
 }
 
 // Trilinear density at voxel-unit position (cx, cy, cz) (texel centres at i + 0.5, zero outside
-// the volume) from the corner records (nbx, nby: bricks along x and y).
+// the volume) from the corner records (nbx, nby: bricks along x and y).  The last record read stays
+// in registers: consecutive samples of a ray are a fraction of a voxel apart (four per slice), so
+// most fall in the base voxel of the one before and need no load; a lane loads only when its base
+// voxel changes.  (r, lo, hi): the record held, r = ~0 for none.
 __device__ __forceinline__ float density_at(const float4* __restrict__ rec, int X, int Y, int Z, uint32_t nbx,
-                                            uint32_t nby, float cx, float cy, float cz) {
+                                            uint32_t nby, float cx, float cy, float cz, uint64_t& r, float4& lo,
+                                            float4& hi) {
     const float xb = cx - 0.5f, yb = cy - 0.5f, zb = cz - 0.5f;
     const float fx = floorf(xb), fy = floorf(yb), fz = floorf(zb);
-    // outside: every corner is in the zero border or beyond (and the record index stays in range)
     if (!(fx > -2.0f && fx < (float)X && fy > -2.0f && fy < (float)Y && fz > -2.0f && fz < (float)Z)) return 0.0f;
     const float ax = xb - fx, ay = yb - fy, az = zb - fz;
-    const uint64_t r = corner_record((uint32_t)((int)fx + 1), (uint32_t)((int)fy + 1), (uint32_t)((int)fz + 1), nbx, nby);
-    const float4 lo = rec[2 * r], hi = rec[2 * r + 1];
+    const uint64_t rn =
+        corner_record((uint32_t)((int)fx + 1), (uint32_t)((int)fy + 1), (uint32_t)((int)fz + 1), nbx, nby);
+    if (rn != r) {
+        lo = rec[2 * rn];
+        hi = rec[2 * rn + 1];
+        r = rn;
+    }
     const float c00 = fmaf(ax, lo.y - lo.x, lo.x);
     const float c10 = fmaf(ax, lo.w - lo.z, lo.z);
     const float c01 = fmaf(ax, hi.y - hi.x, hi.x);
@@ -86,7 +94,7 @@ __device__ __forceinline__ float density_at(const float4* __restrict__ rec, int 
 // accumulate the radiological path in FP64 (0: every beamlet; infinity: none).
 // Workgroup (beamlet blockIdx.x, ray block blockIdx.y): `rays` consecutive BEV pixels (a multiple
 // of 64) times blockDim.x / rays segments.
-__global__ __launch_bounds__(kTermaThreads) void terma_kernel(
+__global__ __launch_bounds__(kTermaThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void terma_kernel(
     const BeamletDev* __restrict__ beams, const float* __restrict__ fluence, const float* __restrict__ density_pad,
     int X, int Y, int Z, float vsx, float vsy, float vsz, SpectrumDev spec, float rpl64_len,
     float* __restrict__ terma, float* __restrict__ dens_bev, float4* __restrict__ scratch, int rays) {
@@ -119,9 +127,12 @@ __global__ __launch_bounds__(kTermaThreads) void terma_kernel(
 
     const uint32_t nbx = ((uint32_t)X + 4) / 4, nby = ((uint32_t)Y + 4) / 4;
     const float4* __restrict__ rec = reinterpret_cast<const float4*>(density_pad);
+    uint64_t rc = ~uint64_t(0);
+    float4 rlo = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rhi = rlo;
     auto rho_at = [&](int k) {
         const float t = fmaf((float)k, dt, t0);
-        return density_at(rec, X, Y, Z, nbx, nby, fmaf(ax, t, sx), fmaf(ay, t, sy), fmaf(az, t, sz));
+        return density_at(rec, X, Y, Z, nbx, nby, fmaf(ax, t, sx), fmaf(ay, t, sy), fmaf(az, t, sz), rc, rlo,
+                                 rhi);
     };
 
     float* __restrict__ T = terma + B.bev_off + p;
