@@ -36,9 +36,11 @@ constexpr int kTermaThreads = 1024;  // workgroup size limit: ray segments x ray
 constexpr int kTermaMaxSeg = 16;    // most ray segments (16: four workgroups of 64 rays per beamlet)
 
 // Corner records: for every base voxel (x, y, z) in [0, X] x [0, Y] x [0, Z] of the density with a
-// one-voxel zero border, its eight trilinear corners (x..x+1, y..y+1, z..z+1; x fastest) as two
-// float4, the records in 4 x 4 x 4 bricks: record ((z/4 nby + y/4) nbx + x/4) 64 + 16 (z%4) +
-// 4 (y%4) + x%4, nb* = ceil((dim + 1) / 4).  A sample reads one 32-byte record.
+// one-voxel zero border, its eight trilinear corners c0..c7 (x..x+1, y..y+1, z..z+1; x fastest) as
+// two float4 (c0, c1 - c0, c2, c3 - c2), (c4, c5 - c4, c6, c7 - c6): the x differences the first
+// interpolation step needs, rounded as it would round them.  Records in 4 x 4 x 4 bricks: record
+// ((z/4 nby + y/4) nbx + x/4) 64 + 16 (z%4) + 4 (y%4) + x%4, nb* = ceil((dim + 1) / 4).  A sample
+// reads one 32-byte record.
 __device__ __forceinline__ uint64_t corner_record(uint32_t x, uint32_t y, uint32_t z, uint32_t nbx, uint32_t nby) {
     return ((((uint64_t)(z >> 2) * nby + (y >> 2)) * nbx + (x >> 2)) << 6) + ((z & 3) << 4) + ((y & 3) << 2) + (x & 3);
 }
@@ -58,8 +60,8 @@ __global__ __launch_bounds__(256) void expand_density_kernel(const float* __rest
         v[c] = in ? density[((size_t)oz * Y + oy) * X + ox] : 0.0f;
     }
     const uint64_t r = corner_record(x, y, z, nbx, nby);
-    out[2 * r] = make_float4(v[0], v[1], v[2], v[3]);
-    out[2 * r + 1] = make_float4(v[4], v[5], v[6], v[7]);
+    out[2 * r] = make_float4(v[0], v[1] - v[0], v[2], v[3] - v[2]);
+    out[2 * r + 1] = make_float4(v[4], v[5] - v[4], v[6], v[7] - v[6]);
 }
 
 // Trilinear density at voxel-unit position (cx, cy, cz) (texel centres at i + 0.5, zero outside
@@ -72,6 +74,7 @@ __device__ __forceinline__ float density_at(const float4* __restrict__ rec, int 
                                             float4& hi) {
     const float xb = cx - 0.5f, yb = cy - 0.5f, zb = cz - 0.5f;
     const float fx = floorf(xb), fy = floorf(yb), fz = floorf(zb);
+    // outside: every corner is in the zero border or beyond (and the record index stays in range)
     if (!(fx > -2.0f && fx < (float)X && fy > -2.0f && fy < (float)Y && fz > -2.0f && fz < (float)Z)) return 0.0f;
     const float ax = xb - fx, ay = yb - fy, az = zb - fz;
     const uint64_t rn =
@@ -81,10 +84,10 @@ __device__ __forceinline__ float density_at(const float4* __restrict__ rec, int 
         hi = rec[2 * rn + 1];
         r = rn;
     }
-    const float c00 = fmaf(ax, lo.y - lo.x, lo.x);
-    const float c10 = fmaf(ax, lo.w - lo.z, lo.z);
-    const float c01 = fmaf(ax, hi.y - hi.x, hi.x);
-    const float c11 = fmaf(ax, hi.w - hi.z, hi.z);
+    const float c00 = fmaf(ax, lo.y, lo.x);
+    const float c10 = fmaf(ax, lo.w, lo.z);
+    const float c01 = fmaf(ax, hi.y, hi.x);
+    const float c11 = fmaf(ax, hi.w, hi.z);
     const float c0 = fmaf(ay, c10 - c00, c00);
     const float c1 = fmaf(ay, c11 - c01, c01);
     return fmaf(az, c1 - c0, c0);
