@@ -132,10 +132,11 @@ __global__ __launch_bounds__(kRowThreads) void resample_count_kernel(const Beaml
     const int z0 = blockIdx.x * kCountPlanes, b = blockIdx.y;
     const BeamletDev& B = beams[b];
     const double rho = rho_of(vsx, vsy, vsz);
-    // Planes whose voxel centres lie outside the z-range of the widened frustum (the extremes of
-    // its eight vertices, mapped back with the transpose of the rotation, plus 10 um for rounding)
-    // hold no candidate: only the planes in [za, zb) are walked row by row.
-    double zmin = 1e300, zmax = -1e300;
+    // Planes (rows) whose voxel centres lie outside the z-range (y-range) of the widened frustum
+    // (the extremes of its eight vertices, mapped back with the transpose of the rotation, plus
+    // 10 um for rounding) hold no candidate: only the rows [ya, yb) of the planes in [za, zb) are
+    // walked.
+    double zmin = 1e300, zmax = -1e300, ymin = 1e300, ymax = -1e300;
     {
         const double cx = 0.5 * kFmap * (double)B.bs[0] / (double)B.sad;
         const double cz = 0.5 * kFmap * (double)B.bs[1] / (double)B.sad;
@@ -147,21 +148,26 @@ __global__ __launch_bounds__(kRowThreads) void resample_count_kernel(const Beaml
             const double bx = ((v & 2) ? 1.0 : -1.0) * fabs(cx * by + rx);
             const double bz = ((v & 4) ? 1.0 : -1.0) * fabs(cz * by + rz);
             const double qz = (double)B.src[2] + (double)B.Rf[2] * bx + (double)B.Rf[5] * by + (double)B.Rf[8] * bz;
+            const double qy = (double)B.src[1] + (double)B.Rf[1] * bx + (double)B.Rf[4] * by + (double)B.Rf[7] * bz;
             zmin = fmin(zmin, qz);
             zmax = fmax(zmax, qz);
+            ymin = fmin(ymin, qy);
+            ymax = fmax(ymax, qy);
         }
     }
     const int za = max(z0, (int)floor((zmin - 1e-3) / (double)vsz - 0.5));
     const int zb = min(min(z0 + kCountPlanes, Z), (int)ceil((zmax + 1e-3) / (double)vsz - 0.5) + 1);
+    const int ya = max(0, (int)floor((ymin - 1e-3) / (double)vsy - 0.5));
+    const int yb = min(Y, (int)ceil((ymax + 1e-3) / (double)vsy - 0.5) + 1);
     if (threadIdx.x < kCountPlanes) {
         s_cnt[threadIdx.x] = 0u;
         s_ylo[threadIdx.x] = 0xFFFFFFFFu;
         s_yhi[threadIdx.x] = 0u;
     }
     __syncthreads();
-    const int np = max(zb - za, 0);
-    for (int i = threadIdx.x; i < np * Y; i += blockDim.x) {
-        const int p = i / Y, y = i - p * Y;
+    const int np = max(zb - za, 0), ny = max(yb - ya, 0);
+    for (int i = threadIdx.x; i < np * ny; i += blockDim.x) {
+        const int p = i / ny, y = ya + (i - p * ny);
         int x0, x1;
         row_interval(B, y, za + p, X, vsx, vsy, vsz, rho, x0, x1);
         if (x1 > x0) {
