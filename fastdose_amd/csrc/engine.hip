@@ -125,11 +125,11 @@ int max_long_dim(const fd_beam* beams, int n) {
 
 namespace {
 
-// Device descriptors of n beamlets with BEV volumes `stride` floats apart.  Beamlets of one beam
-// share the orientation, so the rotation matrices are computed once per run of equal angles.
-void build_descriptors(const fd_beam* beams, int n, uint64_t stride, BeamletDev* out) {
-    for (int i = 0; i < n; i++) {
-        if (i > 0 && std::memcmp(beams[i].angles, beams[i - 1].angles, sizeof(beams[i].angles)) == 0) {
+// Device descriptors of beamlets [i0, i1) with BEV volumes `stride` floats apart.  Beamlets of one
+// beam share the orientation, so the rotation matrices are computed once per run of equal angles.
+void build_descriptors(const fd_beam* beams, int i0, int i1, uint64_t stride, BeamletDev* out) {
+    for (int i = i0; i < i1; i++) {
+        if (i > i0 && std::memcmp(beams[i].angles, beams[i - 1].angles, sizeof(beams[i].angles)) == 0) {
             std::memcpy(out[i].Rf, out[i - 1].Rf, sizeof(out[i].Rf));
             std::memcpy(out[i].Ri, out[i - 1].Ri, sizeof(out[i].Ri));
             fill_beamlet_dev_no_rotation(beams[i], (uint64_t)i * stride, out[i]);
@@ -139,24 +139,14 @@ void build_descriptors(const fd_beam* beams, int n, uint64_t stride, BeamletDev*
     }
 }
 
-// memcpy of a large host buffer in up to 8 parts on threads of their own (one core copies into
-// pinned memory at a few GB/s: 16 MiB of fluence took about 2 ms on one)
-void copy_parallel(void* dst, const void* src, size_t bytes) {
-    constexpr size_t kPart = size_t(4) << 20;
-    const size_t parts = std::min<size_t>(8, bytes / kPart);
-    if (parts < 2) {
-        std::memcpy(dst, src, bytes);
-        return;
-    }
-    const size_t step = (bytes / parts + 63) & ~size_t(63);
+// fn(part) for part = 0 .. parts - 1, parts > 1 of them on threads of their own.
+template <class F>
+void run_parts(int parts, const F& fn) {
     std::thread th[8];
-    for (size_t p = 1; p < parts; p++) {
-        const size_t a = p * step, b = std::min(bytes, a + step);
-        if (a < b) th[p] = std::thread([=] { std::memcpy((char*)dst + a, (const char*)src + a, b - a); });
-    }
-    std::memcpy(dst, src, std::min(bytes, step));
-    for (size_t p = 1; p < parts; p++)
-        if (th[p].joinable()) th[p].join();
+    parts = std::min(parts, 8);
+    for (int p = 1; p < parts; p++) th[p] = std::thread([&fn, p] { fn(p); });
+    fn(0);
+    for (int p = 1; p < parts; p++) th[p].join();
 }
 
 // Pinned staging of the descriptors (beams non-null) and optionally the fluence of a batch; waits
@@ -170,18 +160,24 @@ int stage_inputs(Workspace& w, const fd_beam* beams, int n, uint64_t stride, con
         FD_HIP(hipHostMalloc((void**)&w.stage, (size_t)n * sizeof(BeamletDev), hipHostMallocDefault));
         w.stage_cap = n;
     }
-    if (beams) build_descriptors(beams, n, stride, w.stage);
-    if (fluence_host) {
-        const size_t nf = (size_t)n * kNpix;
-        if (nf > w.flu_cap) {
-            if (w.flu_stage) FD_HIP(hipHostFree(w.flu_stage));
-            w.flu_stage = nullptr;
-            w.flu_cap = 0;
-            FD_HIP(hipHostMalloc((void**)&w.flu_stage, nf * sizeof(float), hipHostMallocDefault));
-            w.flu_cap = nf;
-        }
-        copy_parallel(w.flu_stage, fluence_host, nf * sizeof(float));
+    const size_t nf = fluence_host ? (size_t)n * kNpix : 0;
+    if (nf > w.flu_cap) {
+        if (w.flu_stage) FD_HIP(hipHostFree(w.flu_stage));
+        w.flu_stage = nullptr;
+        w.flu_cap = 0;
+        FD_HIP(hipHostMalloc((void**)&w.flu_stage, nf * sizeof(float), hipHostMallocDefault));
+        w.flu_cap = nf;
     }
+    // descriptors (about 0.15 us each, the rotation matrices) and the fluence copy (one core
+    // copies into pinned memory at a few GB/s) split over up to 8 threads for large batches
+    const int parts = n >= 2048 ? 8 : 1;
+    run_parts(parts, [&](int p) {
+        const int i0 = (int)((int64_t)n * p / parts), i1 = (int)((int64_t)n * (p + 1) / parts);
+        if (beams) build_descriptors(beams, i0, i1, stride, w.stage);
+        if (nf)
+            std::memcpy(w.flu_stage + (size_t)i0 * kNpix, fluence_host + (size_t)i0 * kNpix,
+                        (size_t)(i1 - i0) * kNpix * sizeof(float));
+    });
     return FD_OK;
 }
 
@@ -225,7 +221,7 @@ int upload_beams(Workspace& w, const fd_beam* beams, int n, uint64_t stride, hip
     // device already holds these exact descriptors, ordered on this stream or already copied, the
     // upload (and the copy engine's hand-off between two kernels) is skipped.
     w.desc_next.resize((size_t)n);
-    build_descriptors(beams, n, stride, w.desc_next.data());
+    build_descriptors(beams, 0, n, stride, w.desc_next.data());
     const size_t bytes = (size_t)n * sizeof(BeamletDev);
     if (w.desc_valid && w.desc_host.size() == (size_t)n &&
         std::memcmp(w.desc_host.data(), w.desc_next.data(), bytes) == 0 &&
