@@ -130,6 +130,9 @@ struct Workspace {
     bool timing = false;
     hipGraphExec_t gexec = nullptr;   // last instantiated batch graph (FD_OPT_GRAPH)
     hipStream_t own_stream = nullptr; // capture stream when the caller passes the null stream
+    // second stream of a batch: the CCCS march tables are built there while Terma runs
+    hipStream_t aux_stream = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
 
     int init();
     void release();
@@ -220,8 +223,11 @@ int max_long_dim(const fd_beam* beams, int n);
 int upload_beams(Workspace& w, const fd_beam* beams, int n, uint64_t stride, hipStream_t stream);
 int run_terma(const fd_ctx* ctx, Workspace& w, int options, int n, const float* fluence, float* terma, float* dens,
               uint64_t stride, hipStream_t stream);
+// The march tables `tab` (timing slot FD_T_CCCS_TABLES) on `stream`.
+int run_cccs_tables(const fd_ctx* ctx, Workspace& w, const CccsTables& tab, hipStream_t stream);
 int run_cccs(const fd_ctx* ctx, Workspace& w, int options, int n, int max_L, const float* terma, const float* dens,
-             float* dose, uint64_t stride, hipStream_t stream, const CccsTables* tab = nullptr);
+             float* dose, uint64_t stride, hipStream_t stream, const CccsTables* tab = nullptr,
+             bool tab_built = false);
 // The whole resampler into the caller's CSR arrays `out` (offsets always; columns and values
 // only when nnz <= out.capacity), with one host synchronisation (a second only when the estimated
 // staging overflows).  thr > 0 keeps only entries >= thr * (the beamlet's maximum value).

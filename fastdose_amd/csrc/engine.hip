@@ -28,6 +28,11 @@ void Workspace::release() {
     gexec = nullptr;
     if (own_stream) (void)hipStreamDestroy(own_stream);
     own_stream = nullptr;
+    if (aux_stream) (void)hipStreamDestroy(aux_stream);
+    aux_stream = nullptr;
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    if (join_ev) (void)hipEventDestroy(join_ev);
+    fork_ev = join_ev = nullptr;
     if (stage) (void)hipHostFree(stage);
     if (flu_stage) (void)hipHostFree(flu_stage);
     if (tab_stage) (void)hipHostFree(tab_stage);
@@ -266,18 +271,21 @@ int reserve_cccs(const fd_ctx* ctx, Workspace& w, int n, uint64_t stride) {
     return FD_OK;
 }
 
-int run_cccs(const fd_ctx* ctx, Workspace& w, int options, int n, int max_L, const float* terma, const float* dens,
-             float* dose, uint64_t stride, hipStream_t stream, const CccsTables* tab) {
-    FD_TRY(reserve_cccs(ctx, w, n, stride));
-    if (tab) {
-        w.mark(FD_T_CCCS_TABLES, 0, stream);
-        const hipError_t et = launch_cccs_tables(*tab, ctx->cones_dev, ctx->n_fwd, ctx->cones.n_phi, stream);
-        w.mark(FD_T_CCCS_TABLES, 1, stream);
-        if (et != hipSuccess) {
-            set_error(std::string("cccs tables launch: ") + hipGetErrorString(et));
-            return FD_ERR_HIP;
-        }
+int run_cccs_tables(const fd_ctx* ctx, Workspace& w, const CccsTables& tab, hipStream_t stream) {
+    w.mark(FD_T_CCCS_TABLES, 0, stream);
+    const hipError_t et = launch_cccs_tables(tab, ctx->cones_dev, ctx->n_fwd, ctx->cones.n_phi, stream);
+    w.mark(FD_T_CCCS_TABLES, 1, stream);
+    if (et != hipSuccess) {
+        set_error(std::string("cccs tables launch: ") + hipGetErrorString(et));
+        return FD_ERR_HIP;
     }
+    return FD_OK;
+}
+
+int run_cccs(const fd_ctx* ctx, Workspace& w, int options, int n, int max_L, const float* terma, const float* dens,
+             float* dose, uint64_t stride, hipStream_t stream, const CccsTables* tab, bool tab_built) {
+    FD_TRY(reserve_cccs(ctx, w, n, stride));
+    if (tab && !tab_built) FD_TRY(run_cccs_tables(ctx, w, *tab, stream));
     hipEvent_t evs[3];
     const bool t = w.timing && w.ev[0];
     if (t) {
@@ -576,6 +584,11 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
     FD_TRY(plan_tables(ctx, w, options, beams, n));
     FD_TRY(stage_inputs(w, beams, n, stride, fluence));
     const CccsTables tabs{w.tab_ent.ptr, w.tab_rec.ptr, w.tab_hdr.ptr, w.flags.ptr + 2, w.tab_n_gt};
+    if (w.tab_n_gt && !w.aux_stream) {
+        FD_HIP(hipStreamCreateWithFlags(&w.aux_stream, hipStreamNonBlocking));
+        FD_HIP(hipEventCreateWithFlags(&w.fork_ev, hipEventDisableTiming));
+        FD_HIP(hipEventCreateWithFlags(&w.join_ev, hipEventDisableTiming));
+    }
     // the batch's CSR entries: into the caller's sink when they fit there, else the workspace
     CsrDst dst1{w.out_cols.ptr, w.out_vals.ptr, cap};
     CsrDst dst0 = sink ? *sink : dst1;
@@ -589,12 +602,20 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
         FD_HIP(hipMemcpyAsync(w.desc.ptr, w.stage, (size_t)n * sizeof(BeamletDev), hipMemcpyHostToDevice, st));
         FD_HIP(hipMemcpyAsync(w.fluence.ptr, w.flu_stage, (size_t)n * kNpix * sizeof(float), hipMemcpyHostToDevice,
                               st));
-        if (w.tab_n_gt)
+        if (w.tab_n_gt) {
+            // the march tables depend on the geometries only: built on the second stream while
+            // Terma runs (a few hundred single-wavefront workgroups would leave the GPU idle)
             FD_HIP(hipMemcpyAsync(w.tab_ent.ptr, w.tab_stage, (size_t)w.tab_n_gt * sizeof(TabEntry),
                                   hipMemcpyHostToDevice, st));
+            FD_HIP(hipEventRecord(w.fork_ev, st));
+            FD_HIP(hipStreamWaitEvent(w.aux_stream, w.fork_ev, 0));
+            FD_TRY(run_cccs_tables(ctx, w, tabs, w.aux_stream));
+            FD_HIP(hipEventRecord(w.join_ev, w.aux_stream));
+        }
         FD_TRY(run_terma(ctx, w, options, n, w.fluence.ptr, w.terma.ptr, w.dens_bev.ptr, stride, st));
+        if (w.tab_n_gt) FD_HIP(hipStreamWaitEvent(st, w.join_ev, 0));
         FD_TRY(run_cccs(ctx, w, options, n, maxL, w.terma.ptr, w.dens_bev.ptr, w.dose_bev.ptr, stride, st,
-                        w.tab_n_gt ? &tabs : nullptr));
+                        w.tab_n_gt ? &tabs : nullptr, true));
         FD_TRY(launch_count(ctx, w, n, st));
         FD_TRY(launch_values(ctx, w, n, w.dose_bev.ptr, extent, thr, cap, st));
         FD_TRY(run_resample_compact(ctx, w, n, thr, w.rowoff.ptr, dst0, dst1, st));
