@@ -130,9 +130,10 @@ struct Workspace {
     bool timing = false;
     hipGraphExec_t gexec = nullptr;   // last instantiated batch graph (FD_OPT_GRAPH)
     hipStream_t own_stream = nullptr; // capture stream when the caller passes the null stream
-    // second stream of a batch: the CCCS march tables are built there while Terma runs
+    // second stream of a batch: the CCCS march tables and the resampler's candidate count run
+    // there while Terma runs (fork, tables done, count done)
     hipStream_t aux_stream = nullptr;
-    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr, count_ev = nullptr;
 
     int init();
     void release();

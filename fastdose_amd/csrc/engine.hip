@@ -32,7 +32,8 @@ void Workspace::release() {
     aux_stream = nullptr;
     if (fork_ev) (void)hipEventDestroy(fork_ev);
     if (join_ev) (void)hipEventDestroy(join_ev);
-    fork_ev = join_ev = nullptr;
+    if (count_ev) (void)hipEventDestroy(count_ev);
+    fork_ev = join_ev = count_ev = nullptr;
     if (stage) (void)hipHostFree(stage);
     if (flu_stage) (void)hipHostFree(flu_stage);
     if (tab_stage) (void)hipHostFree(tab_stage);
@@ -584,10 +585,11 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
     FD_TRY(plan_tables(ctx, w, options, beams, n));
     FD_TRY(stage_inputs(w, beams, n, stride, fluence));
     const CccsTables tabs{w.tab_ent.ptr, w.tab_rec.ptr, w.tab_hdr.ptr, w.flags.ptr + 2, w.tab_n_gt};
-    if (w.tab_n_gt && !w.aux_stream) {
+    if (!w.aux_stream) {
         FD_HIP(hipStreamCreateWithFlags(&w.aux_stream, hipStreamNonBlocking));
         FD_HIP(hipEventCreateWithFlags(&w.fork_ev, hipEventDisableTiming));
         FD_HIP(hipEventCreateWithFlags(&w.join_ev, hipEventDisableTiming));
+        FD_HIP(hipEventCreateWithFlags(&w.count_ev, hipEventDisableTiming));
     }
     // the batch's CSR entries: into the caller's sink when they fit there, else the workspace
     CsrDst dst1{w.out_cols.ptr, w.out_vals.ptr, cap};
@@ -602,21 +604,25 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
         FD_HIP(hipMemcpyAsync(w.desc.ptr, w.stage, (size_t)n * sizeof(BeamletDev), hipMemcpyHostToDevice, st));
         FD_HIP(hipMemcpyAsync(w.fluence.ptr, w.flu_stage, (size_t)n * kNpix * sizeof(float), hipMemcpyHostToDevice,
                               st));
-        if (w.tab_n_gt) {
-            // the march tables depend on the geometries only: built on the second stream while
-            // Terma runs (a few hundred single-wavefront workgroups would leave the GPU idle)
+        // What depends on the geometries only runs on the second stream while Terma runs (Terma
+        // leaves vector issue slots free, and a few hundred single-wavefront table workgroups
+        // would leave the GPU idle): the CCCS march tables, then the resampler's candidate count.
+        if (w.tab_n_gt)
             FD_HIP(hipMemcpyAsync(w.tab_ent.ptr, w.tab_stage, (size_t)w.tab_n_gt * sizeof(TabEntry),
                                   hipMemcpyHostToDevice, st));
-            FD_HIP(hipEventRecord(w.fork_ev, st));
-            FD_HIP(hipStreamWaitEvent(w.aux_stream, w.fork_ev, 0));
+        FD_HIP(hipEventRecord(w.fork_ev, st));
+        FD_HIP(hipStreamWaitEvent(w.aux_stream, w.fork_ev, 0));
+        if (w.tab_n_gt) {
             FD_TRY(run_cccs_tables(ctx, w, tabs, w.aux_stream));
             FD_HIP(hipEventRecord(w.join_ev, w.aux_stream));
         }
+        FD_TRY(launch_count(ctx, w, n, w.aux_stream));
+        FD_HIP(hipEventRecord(w.count_ev, w.aux_stream));
         FD_TRY(run_terma(ctx, w, options, n, w.fluence.ptr, w.terma.ptr, w.dens_bev.ptr, stride, st));
         if (w.tab_n_gt) FD_HIP(hipStreamWaitEvent(st, w.join_ev, 0));
         FD_TRY(run_cccs(ctx, w, options, n, maxL, w.terma.ptr, w.dens_bev.ptr, w.dose_bev.ptr, stride, st,
                         w.tab_n_gt ? &tabs : nullptr, true));
-        FD_TRY(launch_count(ctx, w, n, st));
+        FD_HIP(hipStreamWaitEvent(st, w.count_ev, 0));
         FD_TRY(launch_values(ctx, w, n, w.dose_bev.ptr, extent, thr, cap, st));
         FD_TRY(run_resample_compact(ctx, w, n, thr, w.rowoff.ptr, dst0, dst1, st));
         FD_TRY(fetch_totals(ctx, w, n, st, true));
