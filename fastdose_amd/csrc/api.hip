@@ -753,7 +753,7 @@ int fd_dosemat_add_beams(fd_ctx* ctx, fd_dosemat* m, const fd_beam* beamlets, co
         const uint64_t room = have > m->used_nnz ? have - m->used_nnz : 0;
         const CsrDst sink{m->columns.ptr + m->used_nnz, m->values.ptr + m->used_nnz, room};
         FD_TRY(run_batch(ctx, ctx->ws, options, beamlets + first[k0], (int)n, fluence + first[k0] * kNpix, extent,
-                         ctx->threshold, s, &res, room ? &sink : nullptr));
+                         ctx->threshold, s, &res, room ? &sink : nullptr, true));
         // one matrix per beam: offsets re-based to each beam's first row
         host_off.clear();
         host_off.reserve(n + (k1 - k0));

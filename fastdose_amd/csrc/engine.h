@@ -28,16 +28,20 @@ namespace fdi {
         if (r_ != FD_OK) return r_; \
     } while (0)
 
-// Growable device allocation (contents are not kept when it grows).
+// Growable device allocation (contents are not kept when it grows).  With `stream` set, the
+// allocations and frees are ordered on that stream (hipMallocAsync / hipFreeAsync) instead of
+// synchronising the device: a server's slots grow their scratch while other slots' batches run,
+// and a device-wide free there waits for those batches (and has stalled behind them for seconds).
 template <typename T>
 struct DevBuf {
     T* ptr = nullptr;
     size_t cap = 0;  // elements
+    hipStream_t stream = nullptr;
     int reserve(size_t n) {
         if (n <= cap) return FD_OK;
         size_t want = std::max(n, cap + cap / 2);
         if (ptr) {
-            hipError_t e = hipFree(ptr);
+            hipError_t e = stream ? hipFreeAsync(ptr, stream) : hipFree(ptr);
             ptr = nullptr;
             cap = 0;
             if (e != hipSuccess) {
@@ -45,7 +49,8 @@ struct DevBuf {
                 return FD_ERR_HIP;
             }
         }
-        hipError_t e = hipMalloc((void**)&ptr, want * sizeof(T));
+        hipError_t e = stream ? hipMallocAsync((void**)&ptr, want * sizeof(T), stream)
+                              : hipMalloc((void**)&ptr, want * sizeof(T));
         if (e != hipSuccess) {
             ptr = nullptr;
             set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
@@ -55,11 +60,16 @@ struct DevBuf {
         return FD_OK;
     }
     void release() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr) (void)(stream ? hipFreeAsync(ptr, stream) : hipFree(ptr));
         ptr = nullptr;
         cap = 0;
     }
     size_t bytes() const { return cap * sizeof(T); }
+    // exchange the storage (not the stream) with another buffer ordered on the same device
+    void swap_storage(DevBuf& o) {
+        std::swap(ptr, o.ptr);
+        std::swap(cap, o.cap);
+    }
 };
 
 // Growable device buffer that keeps its first `used` elements (used by the dose-matrix ensemble;
@@ -137,6 +147,21 @@ struct Workspace {
 
     int init();
     void release();
+    // Every device buffer of the workspace allocates and frees in the order of stream s (a server
+    // slot's stream); nullptr: synchronous allocation.
+    void set_alloc_stream(hipStream_t s) {
+        desc.stream = partial.stream = fluence.stream = terma.stream = dens_bev.stream = dose_bev.stream = s;
+        cand_count.stream = nnz_count.stream = flags.stream = rowmax.stream = s;
+        cand_rows.stream = s;
+        cand_off.stream = nnz_off.stream = tmp_cols.stream = rowoff.stream = out_cols.stream = scan_tmp.stream = s;
+        tmp_vals.stream = out_vals.stream = s;
+        tab_ent.stream = s;
+        tab_rec.stream = s;
+        tab_hdr.stream = s;
+    }
+    // Pinned host staging for batches of up to n beamlets (n_fwd forward cones), so that batches
+    // up to that size never free and reallocate it (freeing pinned memory waits for the device).
+    int reserve_staging(int n, int n_fwd);
     void mark(int slot, int which, hipStream_t s) {
         if (!timing || !ev[0]) return;
         if (which == 0) fold(slot);  // the previous launch's pair is about to be re-recorded
@@ -241,8 +266,13 @@ int run_resample_compact(const fd_ctx* ctx, Workspace& w, int n, float thr, uint
 // Terma -> CCCS -> resampling, one host synchronisation (two when the resampler staging estimate
 // was too small).  The CSR columns and values go to *sink when its capacity holds them (the
 // device decides; out->in_sink tells), else to w.out_cols / w.out_vals.
+// overlap: the march tables and the candidate count run on a second stream (w.aux_stream) joined
+// by events, and the batch's staging runs on several host threads.  Only for callers that submit
+// all their device work from one thread: cross-stream waits submitted from several threads at
+// once (a server's slots) can stall behind each other when streams share hardware queues.
 int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams, int n, const float* fluence,
-              float extent, float thr, hipStream_t stream, BatchResult* out, const CsrDst* sink = nullptr);
+              float extent, float thr, hipStream_t stream, BatchResult* out, const CsrDst* sink = nullptr,
+              bool overlap = false);
 
 // Device bytes a batch of these beamlets needs (BEV volumes, cone partials, resampler staging).
 size_t batch_footprint(const fd_ctx* ctx, const fd_beam* beams, int n);

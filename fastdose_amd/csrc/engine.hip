@@ -14,6 +14,20 @@
 
 namespace fdi {
 
+// Pinned host buffer of at least `need` elements (grows to twice its size at least: freeing pinned
+// memory waits for the device).
+template <typename T>
+int grow_pinned(T*& p, size_t& cap, size_t need) {
+    if (need <= cap) return FD_OK;
+    const size_t want = std::max(need, 2 * cap);
+    if (p) FD_HIP(hipHostFree(p));
+    p = nullptr;
+    cap = 0;
+    FD_HIP(hipHostMalloc((void**)&p, want * sizeof(T), hipHostMallocDefault));
+    cap = want;
+    return FD_OK;
+}
+
 int Workspace::init() {
     FD_HIP(hipEventCreateWithFlags(&stage_done, hipEventDisableTiming));
     FD_HIP(hipHostMalloc((void**)&host_totals, 4 * sizeof(uint64_t), hipHostMallocDefault));
@@ -96,6 +110,14 @@ size_t Workspace::footprint() const {
            rowmax.bytes() + tab_ent.bytes() + tab_rec.bytes() + tab_hdr.bytes();
 }
 
+int Workspace::reserve_staging(int n, int n_fwd) {
+    FD_TRY(grow_pinned(stage, stage_cap, (size_t)n));
+    FD_TRY(grow_pinned(flu_stage, flu_cap, (size_t)n * kNpix));
+    FD_TRY(grow_pinned(host_rowoff, host_rowoff_cap, (size_t)n + 1));
+    FD_TRY(grow_pinned(tab_stage, tab_stage_cap, (size_t)n * std::max(n_fwd, 1)));
+    return FD_OK;
+}
+
 int check_beams(const fd_beam* beams, int n) {
     if (n < 0 || (n > 0 && !beams)) {
         set_error("invalid beamlet array");
@@ -157,26 +179,15 @@ void run_parts(int parts, const F& fn) {
 
 // Pinned staging of the descriptors (beams non-null) and optionally the fluence of a batch; waits
 // until the previous upload from this workspace has left the staging area.
-int stage_inputs(Workspace& w, const fd_beam* beams, int n, uint64_t stride, const float* fluence_host) {
+int stage_inputs(Workspace& w, const fd_beam* beams, int n, uint64_t stride, const float* fluence_host,
+                 bool threads = false) {
     FD_HIP(hipEventSynchronize(w.stage_done));
-    if ((size_t)n > w.stage_cap) {
-        if (w.stage) FD_HIP(hipHostFree(w.stage));
-        w.stage = nullptr;
-        w.stage_cap = 0;
-        FD_HIP(hipHostMalloc((void**)&w.stage, (size_t)n * sizeof(BeamletDev), hipHostMallocDefault));
-        w.stage_cap = n;
-    }
+    FD_TRY(grow_pinned(w.stage, w.stage_cap, (size_t)n));
     const size_t nf = fluence_host ? (size_t)n * kNpix : 0;
-    if (nf > w.flu_cap) {
-        if (w.flu_stage) FD_HIP(hipHostFree(w.flu_stage));
-        w.flu_stage = nullptr;
-        w.flu_cap = 0;
-        FD_HIP(hipHostMalloc((void**)&w.flu_stage, nf * sizeof(float), hipHostMallocDefault));
-        w.flu_cap = nf;
-    }
+    FD_TRY(grow_pinned(w.flu_stage, w.flu_cap, nf));
     // descriptors (about 0.15 us each, the rotation matrices) and the fluence copy (one core
     // copies into pinned memory at a few GB/s) split over up to 8 threads for large batches
-    const int parts = n >= 2048 ? 8 : 1;
+    const int parts = threads && n >= 2048 ? 8 : 1;
     run_parts(parts, [&](int p) {
         const int i0 = (int)((int64_t)n * p / parts), i1 = (int)((int64_t)n * (p + 1) / parts);
         if (beams) build_descriptors(beams, i0, i1, stride, w.stage);
@@ -369,15 +380,7 @@ int launch_values(const fd_ctx* ctx, Workspace& w, int n, const float* dose, flo
     return FD_OK;
 }
 
-int reserve_host_rowoff(Workspace& w, int n) {
-    if ((size_t)n + 1 <= w.host_rowoff_cap) return FD_OK;
-    if (w.host_rowoff) FD_HIP(hipHostFree(w.host_rowoff));
-    w.host_rowoff = nullptr;
-    w.host_rowoff_cap = 0;
-    FD_HIP(hipHostMalloc((void**)&w.host_rowoff, ((size_t)n + 1) * sizeof(uint64_t), hipHostMallocDefault));
-    w.host_rowoff_cap = (size_t)n + 1;
-    return FD_OK;
-}
+int reserve_host_rowoff(Workspace& w, int n) { return grow_pinned(w.host_rowoff, w.host_rowoff_cap, (size_t)n + 1); }
 
 // totals of the last resampler pass -> pinned host: [0] candidates, [1] nonzeros, [2] overflow;
 // with rows, also the batch's row offsets -> w.host_rowoff (reserved by the caller)
@@ -504,13 +507,7 @@ int plan_tables(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* bea
     const int ng = (int)geoms.size();
     if (!(options & FD_OPT_TABLES) && n < kTabMinShare * ng) return FD_OK;
     const size_t n_gt = (size_t)ng * n_fwd;
-    if (n_gt > w.tab_stage_cap) {
-        if (w.tab_stage) FD_HIP(hipHostFree(w.tab_stage));
-        w.tab_stage = nullptr;
-        w.tab_stage_cap = 0;
-        FD_HIP(hipHostMalloc((void**)&w.tab_stage, n_gt * sizeof(TabEntry), hipHostMallocDefault));
-        w.tab_stage_cap = n_gt;
-    }
+    FD_TRY(grow_pinned(w.tab_stage, w.tab_stage_cap, n_gt));
     const int chunk = cccs_chunk_steps();
     uint64_t rec = 0, hdr = 0;
     for (int g = 0; g < ng; g++) {
@@ -556,7 +553,7 @@ int plan_tables(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* bea
 }  // namespace
 
 int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams, int n, const float* fluence,
-              float extent, float thr, hipStream_t stream, BatchResult* out, const CsrDst* sink) {
+              float extent, float thr, hipStream_t stream, BatchResult* out, const CsrDst* sink, bool overlap) {
     out->rowoff.assign(1, 0);
     out->nnz = 0;
     if (n == 0) return FD_OK;
@@ -583,9 +580,9 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
     FD_TRY(reserve_cccs(ctx, w, n, stride));
     FD_TRY(reserve_host_rowoff(w, n));
     FD_TRY(plan_tables(ctx, w, options, beams, n));
-    FD_TRY(stage_inputs(w, beams, n, stride, fluence));
+    FD_TRY(stage_inputs(w, beams, n, stride, fluence, overlap));
     const CccsTables tabs{w.tab_ent.ptr, w.tab_rec.ptr, w.tab_hdr.ptr, w.flags.ptr + 2, w.tab_n_gt};
-    if (!w.aux_stream) {
+    if (overlap && !w.aux_stream) {
         FD_HIP(hipStreamCreateWithFlags(&w.aux_stream, hipStreamNonBlocking));
         FD_HIP(hipEventCreateWithFlags(&w.fork_ev, hipEventDisableTiming));
         FD_HIP(hipEventCreateWithFlags(&w.join_ev, hipEventDisableTiming));
@@ -604,25 +601,31 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
         FD_HIP(hipMemcpyAsync(w.desc.ptr, w.stage, (size_t)n * sizeof(BeamletDev), hipMemcpyHostToDevice, st));
         FD_HIP(hipMemcpyAsync(w.fluence.ptr, w.flu_stage, (size_t)n * kNpix * sizeof(float), hipMemcpyHostToDevice,
                               st));
-        // What depends on the geometries only runs on the second stream while Terma runs (Terma
-        // leaves vector issue slots free, and a few hundred single-wavefront table workgroups
-        // would leave the GPU idle): the CCCS march tables, then the resampler's candidate count.
         if (w.tab_n_gt)
             FD_HIP(hipMemcpyAsync(w.tab_ent.ptr, w.tab_stage, (size_t)w.tab_n_gt * sizeof(TabEntry),
                                   hipMemcpyHostToDevice, st));
-        FD_HIP(hipEventRecord(w.fork_ev, st));
-        FD_HIP(hipStreamWaitEvent(w.aux_stream, w.fork_ev, 0));
-        if (w.tab_n_gt) {
-            FD_TRY(run_cccs_tables(ctx, w, tabs, w.aux_stream));
-            FD_HIP(hipEventRecord(w.join_ev, w.aux_stream));
+        if (overlap) {
+            // What depends on the geometries only runs on the second stream while Terma runs
+            // (Terma leaves vector issue slots free, and a few hundred single-wavefront table
+            // workgroups would leave the GPU idle): the CCCS march tables, then the resampler's
+            // candidate count.
+            FD_HIP(hipEventRecord(w.fork_ev, st));
+            FD_HIP(hipStreamWaitEvent(w.aux_stream, w.fork_ev, 0));
+            if (w.tab_n_gt) {
+                FD_TRY(run_cccs_tables(ctx, w, tabs, w.aux_stream));
+                FD_HIP(hipEventRecord(w.join_ev, w.aux_stream));
+            }
+            FD_TRY(launch_count(ctx, w, n, w.aux_stream));
+            FD_HIP(hipEventRecord(w.count_ev, w.aux_stream));
         }
-        FD_TRY(launch_count(ctx, w, n, w.aux_stream));
-        FD_HIP(hipEventRecord(w.count_ev, w.aux_stream));
         FD_TRY(run_terma(ctx, w, options, n, w.fluence.ptr, w.terma.ptr, w.dens_bev.ptr, stride, st));
-        if (w.tab_n_gt) FD_HIP(hipStreamWaitEvent(st, w.join_ev, 0));
+        if (overlap && w.tab_n_gt) FD_HIP(hipStreamWaitEvent(st, w.join_ev, 0));
         FD_TRY(run_cccs(ctx, w, options, n, maxL, w.terma.ptr, w.dens_bev.ptr, w.dose_bev.ptr, stride, st,
-                        w.tab_n_gt ? &tabs : nullptr, true));
-        FD_HIP(hipStreamWaitEvent(st, w.count_ev, 0));
+                        w.tab_n_gt ? &tabs : nullptr, overlap));
+        if (overlap)
+            FD_HIP(hipStreamWaitEvent(st, w.count_ev, 0));
+        else
+            FD_TRY(launch_count(ctx, w, n, st));
         FD_TRY(launch_values(ctx, w, n, w.dose_bev.ptr, extent, thr, cap, st));
         FD_TRY(run_resample_compact(ctx, w, n, thr, w.rowoff.ptr, dst0, dst1, st));
         FD_TRY(fetch_totals(ctx, w, n, st, true));

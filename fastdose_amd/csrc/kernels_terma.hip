@@ -134,8 +134,14 @@ __global__ __launch_bounds__(kTermaThreads) __attribute__((amdgpu_waves_per_eu(8
     float4 rlo = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rhi = rlo;
     auto rho_at = [&](int k) {
         const float t = fmaf((float)k, dt, t0);
-        return density_at(rec, X, Y, Z, nbx, nby, fmaf(ax, t, sx), fmaf(ay, t, sy), fmaf(az, t, sz), rc, rlo,
-                                 rhi);
+        return density_at(rec, X, Y, Z, nbx, nby, fmaf(ax, t, sx), fmaf(ay, t, sy), fmaf(az, t, sz), rc, rlo, rhi);
+    };
+    // the FP64 path's second walk keeps no record in registers (fewer live registers there)
+    auto rho_fresh = [&](int k) {
+        const float t = fmaf((float)k, dt, t0);
+        uint64_t r = ~uint64_t(0);
+        float4 lo, hi;
+        return density_at(rec, X, Y, Z, nbx, nby, fmaf(ax, t, sx), fmaf(ay, t, sy), fmaf(az, t, sz), r, lo, hi);
     };
 
     float* __restrict__ T = terma + B.bev_off + p;
@@ -194,7 +200,7 @@ __global__ __launch_bounds__(kTermaThreads) __attribute__((amdgpu_waves_per_eu(8
             const float t = fmaf((float)k, dt, t0);
             float path;
             if (RPL64) {
-                rpl64 += (double)rho_at(k) * (double)step_len;
+                rpl64 += (double)rho_fresh(k) * (double)step_len;
                 path = (float)rpl64;
             } else {
                 path = seg == 0 ? q[j] : rpl + q[j];

@@ -123,7 +123,9 @@ Block* take_block(fd_server* s) {
         return b;
     }
     s->blocks.push_back(std::make_unique<Block>());
-    return s->blocks.back().get();
+    Block* b = s->blocks.back().get();
+    b->cols.stream = b->vals.stream = s->fetch_stream;  // a block's storage is freed on the fetch stream
+    return b;
 }
 
 void slot_loop(fd_server* s, int slot) {
@@ -185,8 +187,8 @@ void slot_loop(fd_server* s, int slot) {
             Block* blk = nullptr;
             if (rc == FD_OK) {
                 blk = take_block(s);
-                std::swap(blk->cols, w.out_cols);
-                std::swap(blk->vals, w.out_vals);
+                blk->cols.swap_storage(w.out_cols);
+                blk->vals.swap_storage(w.out_vals);
                 blk->rowoff.swap(res.rowoff);
                 blk->beams_left = (int)batch.size();
             }
@@ -263,14 +265,31 @@ int fd_serve_start(fd_ctx* ctx, const fd_serve_opts* opts, fd_server** out) {
     s->t0 = Clock::now();
     s->ws.resize(o.n_slots);
     s->streams.assign(o.n_slots, nullptr);
+    {
+        // slot scratch is allocated and freed in stream order (DevBuf); the device's pool keeps
+        // what the slots free for reuse (by later batches and later servers) instead of returning
+        // it to the system: allocating from the system while another slot's batch runs has stalled
+        // both for seconds
+        hipMemPool_t pool = nullptr;
+        uint64_t keep = UINT64_MAX;
+        if (hipDeviceGetDefaultMemPool(&pool, ctx->device) == hipSuccess && pool)
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
     for (int i = 0; i < o.n_slots; i++) {
         int r = s->ws[i].init();
         hipError_t e = r == FD_OK ? hipStreamCreateWithFlags(&s->streams[i], hipStreamNonBlocking) : hipSuccess;
+        if (r == FD_OK && e == hipSuccess) {
+            s->ws[i].set_alloc_stream(s->streams[i]);
+            r = s->ws[i].reserve_staging((int)o.max_batch_beamlets, ctx->n_fwd);
+        }
         if (r != FD_OK || e != hipSuccess) {
             if (e != hipSuccess) set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
             for (int j = 0; j <= i; j++) {
                 s->ws[j].release();
-                if (s->streams[j]) (void)hipStreamDestroy(s->streams[j]);
+                if (s->streams[j]) {
+                    (void)hipStreamSynchronize(s->streams[j]);
+                    (void)hipStreamDestroy(s->streams[j]);
+                }
             }
             delete s;
             return r != FD_OK ? r : FD_ERR_HIP;
@@ -282,6 +301,7 @@ int fd_serve_start(fd_ctx* ctx, const fd_serve_opts* opts, fd_server** out) {
             set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
             for (int j = 0; j < o.n_slots; j++) {
                 s->ws[j].release();
+                (void)hipStreamSynchronize(s->streams[j]);
                 (void)hipStreamDestroy(s->streams[j]);
             }
             delete s;
@@ -515,17 +535,20 @@ int fd_serve_stop(fd_server* s) {
     (void)hipSetDevice(s->ctx->device);
     for (size_t i = 0; i < s->ws.size(); i++) {
         if (s->streams[i]) (void)hipStreamSynchronize(s->streams[i]);
-        s->ws[i].release();
-        if (s->streams[i]) (void)hipStreamDestroy(s->streams[i]);
+        s->ws[i].release();  // frees ordered on the slot's stream
+        if (s->streams[i]) {
+            (void)hipStreamSynchronize(s->streams[i]);
+            (void)hipStreamDestroy(s->streams[i]);
+        }
+    }
+    // every block, including those of requests never fetched (freed on the fetch stream)
+    for (auto& b : s->blocks) {
+        b->cols.release();
+        b->vals.release();
     }
     if (s->fetch_stream) {
         (void)hipStreamSynchronize(s->fetch_stream);
         (void)hipStreamDestroy(s->fetch_stream);
-    }
-    // every block, including those of requests never fetched
-    for (auto& b : s->blocks) {
-        b->cols.release();
-        b->vals.release();
     }
     delete s;
     return FD_OK;
