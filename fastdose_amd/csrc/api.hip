@@ -435,6 +435,15 @@ int fd_ctx_destroy(fd_ctx* ctx) {
     ctx->ws.release();
     ctx->flag_desc.release();
     ctx->flag_out.release();
+    for (auto& w : ctx->parked_ws) w.release();  // frees ordered on park_stream
+    for (auto& c : ctx->parked_csr) {
+        c.cols.release();
+        c.vals.release();
+    }
+    if (ctx->park_stream) {
+        (void)hipStreamSynchronize(ctx->park_stream);
+        (void)hipStreamDestroy(ctx->park_stream);
+    }
     delete ctx;
     return FD_OK;
 }

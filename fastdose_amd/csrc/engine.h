@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -187,6 +188,12 @@ struct Workspace {
     size_t footprint() const;
 };
 
+// Device CSR entry buffers (column indices and values) of a server's output pool.
+struct CsrBufs {
+    DevBuf<uint64_t> cols;
+    DevBuf<float> vals;
+};
+
 // CSR rows of one finished batch, as the host sees them.
 struct BatchResult {
     std::vector<uint64_t> rowoff;  // n + 1 offsets into the entries' arrays
@@ -218,6 +225,14 @@ struct fd_ctx {
     fdi::Workspace ws;
     fdi::DevBuf<fdi::FlagBeamDev> flag_desc;  // beamlet-flag kernel inputs / outputs
     fdi::DevBuf<uint8_t> flag_out;
+    // Device scratch of stopped servers, kept for the next server on this context: the slots'
+    // workspaces and the output pool's CSR buffers keep their sizes, so the next server's slots
+    // start without allocating (an allocation while another slot's batch runs has stalled both for
+    // seconds).  While parked their buffers are ordered on park_stream; freed with the context.
+    std::mutex park_mu;
+    hipStream_t park_stream = nullptr;
+    std::vector<fdi::Workspace> parked_ws;
+    std::vector<fdi::CsrBufs> parked_csr;
 };
 
 struct fd_dosemat {

@@ -11,7 +11,8 @@
 // caller's ensemble on the server's fetch stream and returns the block to the pool once every beam
 // of its batch has been fetched.  Beams that arrive while
 // others run join the next batch of whichever slot frees first, so their work overlaps on the
-// device with the batches already running.
+// device with the batches already running.  A stopped server parks its slots' workspaces and its
+// output pool on the context, and the next server there starts with them.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -276,7 +277,17 @@ int fd_serve_start(fd_ctx* ctx, const fd_serve_opts* opts, fd_server** out) {
             (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
     }
     for (int i = 0; i < o.n_slots; i++) {
-        int r = s->ws[i].init();
+        // a workspace parked by an earlier server on this context (its buffers keep their sizes)
+        bool parked = false;
+        {
+            std::lock_guard<std::mutex> lk(ctx->park_mu);
+            if (!ctx->parked_ws.empty()) {
+                s->ws[i] = std::move(ctx->parked_ws.back());
+                ctx->parked_ws.pop_back();
+                parked = true;
+            }
+        }
+        int r = parked ? FD_OK : s->ws[i].init();
         hipError_t e = r == FD_OK ? hipStreamCreateWithFlags(&s->streams[i], hipStreamNonBlocking) : hipSuccess;
         if (r == FD_OK && e == hipSuccess) {
             s->ws[i].set_alloc_stream(s->streams[i]);
@@ -307,6 +318,19 @@ int fd_serve_start(fd_ctx* ctx, const fd_serve_opts* opts, fd_server** out) {
             delete s;
             return FD_ERR_HIP;
         }
+    }
+    {
+        // the output pool of an earlier server on this context, as free blocks
+        std::lock_guard<std::mutex> lk(ctx->park_mu);
+        for (CsrBufs& c : ctx->parked_csr) {
+            s->blocks.push_back(std::make_unique<Block>());
+            Block* b = s->blocks.back().get();
+            std::swap(b->cols, c.cols);
+            std::swap(b->vals, c.vals);
+            b->cols.stream = b->vals.stream = s->fetch_stream;
+            s->free_blocks.push_back(b);
+        }
+        ctx->parked_csr.clear();
     }
     for (int i = 0; i < o.n_slots; i++) s->threads.emplace_back(slot_loop, s, i);
     *out = s;
@@ -532,19 +556,51 @@ int fd_serve_stop(fd_server* s) {
     }
     s->cv_work.notify_all();
     for (auto& t : s->threads) t.join();
-    (void)hipSetDevice(s->ctx->device);
+    fd_ctx* ctx = s->ctx;
+    (void)hipSetDevice(ctx->device);
+    // The slots' workspaces and the output pool's buffers are parked on the context for the next
+    // server (without a parking stream they are freed).
+    hipStream_t park = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(ctx->park_mu);
+        if (!ctx->park_stream && hipStreamCreateWithFlags(&ctx->park_stream, hipStreamNonBlocking) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->park_stream = nullptr;
+        }
+        park = ctx->park_stream;
+    }
     for (size_t i = 0; i < s->ws.size(); i++) {
         if (s->streams[i]) (void)hipStreamSynchronize(s->streams[i]);
-        s->ws[i].release();  // frees ordered on the slot's stream
+        Workspace& w = s->ws[i];
+        if (park) {
+            w.set_alloc_stream(park);
+            w.desc_valid = false;
+            w.desc_stream = nullptr;
+            std::lock_guard<std::mutex> lk(ctx->park_mu);
+            ctx->parked_ws.push_back(std::move(w));
+            w = Workspace{};
+        } else {
+            w.release();  // frees ordered on the slot's stream
+        }
         if (s->streams[i]) {
             (void)hipStreamSynchronize(s->streams[i]);
             (void)hipStreamDestroy(s->streams[i]);
         }
     }
-    // every block, including those of requests never fetched (freed on the fetch stream)
+    // every block, including those of requests never fetched
+    if (s->fetch_stream) (void)hipStreamSynchronize(s->fetch_stream);
     for (auto& b : s->blocks) {
-        b->cols.release();
-        b->vals.release();
+        if (park && (b->cols.ptr || b->vals.ptr)) {
+            CsrBufs c;
+            std::swap(c.cols, b->cols);
+            std::swap(c.vals, b->vals);
+            c.cols.stream = c.vals.stream = park;
+            std::lock_guard<std::mutex> lk(ctx->park_mu);
+            ctx->parked_csr.push_back(std::move(c));
+        } else {
+            b->cols.release();  // freed on the fetch stream
+            b->vals.release();
+        }
     }
     if (s->fetch_stream) {
         (void)hipStreamSynchronize(s->fetch_stream);

@@ -383,7 +383,9 @@ int fd_serve_wait(fd_server* s, uint64_t request_id, double timeout_s);
 int fd_serve_fetch(fd_server* s, uint64_t request_id, fd_dosemat* m);
 int fd_serve_stats_get(fd_server* s, fd_serve_stats* out);
 int fd_serve_stats_reset(fd_server* s);
-/* Finishes the queued work, joins the slot threads and frees the server (not the context). */
+/* Finishes the queued work, joins the slot threads and frees the server (not the context).  The
+   slots' device scratch and the output buffers stay with the context for the next server started
+   on it (freed by fd_ctx_destroy). */
 int fd_serve_stop(fd_server* s);
 
 #ifdef __cplusplus
