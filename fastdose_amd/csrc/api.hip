@@ -524,7 +524,7 @@ int fd_terma(fd_ctx* ctx, const fd_beam* beams, int n, const float* fluence_dev,
     }
     hipStream_t s = (hipStream_t)stream;
     FD_TRY(upload_beams(ctx->ws, beams, n, stride, s));
-    return run_terma(ctx, ctx->ws, ctx->options, n, fluence_dev, terma_dev, density_bev_dev, stride, s);
+    return run_terma(ctx, ctx->ws, ctx->options, n, fluence_dev, terma_dev, density_bev_dev, (uint64_t)n * stride, s);
 }
 
 int fd_cccs(fd_ctx* ctx, const fd_beam* beams, int n, const float* terma_dev, const float* density_bev_dev,
@@ -541,7 +541,7 @@ int fd_cccs(fd_ctx* ctx, const fd_beam* beams, int n, const float* terma_dev, co
     hipStream_t s = (hipStream_t)stream;
     FD_TRY(upload_beams(ctx->ws, beams, n, stride, s));
     return run_cccs(ctx, ctx->ws, ctx->options, n, max_long_dim(beams, n), terma_dev, density_bev_dev, dose_bev_dev,
-                    stride, s);
+                    (uint64_t)n * stride, s);
 }
 
 int fd_resample_csr(fd_ctx* ctx, const fd_beam* beams, int n, const float* dose_bev_dev, uint64_t stride,

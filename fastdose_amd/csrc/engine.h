@@ -105,6 +105,7 @@ struct Workspace {
     hipStream_t desc_stream = nullptr;
     DevBuf<float> partial;
     DevBuf<float> fluence, terma, dens_bev, dose_bev;
+    std::vector<uint64_t> bev_offs;  // BEV offsets of the last batch's beamlets, packed (n + 1)
     DevBuf<uint32_t> cand_count, nnz_count;
     DevBuf<uint2> cand_rows;      // per segment: rows [y_lo, y_hi) holding candidates
     DevBuf<uint64_t> cand_off, nnz_off;
@@ -260,14 +261,16 @@ namespace fdi {
 int check_beams(const fd_beam* beams, int n);
 int max_long_dim(const fd_beam* beams, int n);
 
-// Single stages on a workspace (descriptors uploaded by upload_beams first).
+// Single stages on a workspace (descriptors uploaded by upload_beams first: BEV volumes `stride`
+// floats apart).  bev_floats: the floats each of the batch's BEV buffers spans (n x stride for the
+// single stages, the packed total in run_batch), which sizes the per-cone partials.
 int upload_beams(Workspace& w, const fd_beam* beams, int n, uint64_t stride, hipStream_t stream);
 int run_terma(const fd_ctx* ctx, Workspace& w, int options, int n, const float* fluence, float* terma, float* dens,
-              uint64_t stride, hipStream_t stream);
+              uint64_t bev_floats, hipStream_t stream);
 // The march tables `tab` (timing slot FD_T_CCCS_TABLES) on `stream`.
 int run_cccs_tables(const fd_ctx* ctx, Workspace& w, const CccsTables& tab, hipStream_t stream);
 int run_cccs(const fd_ctx* ctx, Workspace& w, int options, int n, int max_L, const float* terma, const float* dens,
-             float* dose, uint64_t stride, hipStream_t stream, const CccsTables* tab = nullptr,
+             float* dose, uint64_t bev_floats, hipStream_t stream, const CccsTables* tab = nullptr,
              bool tab_built = false);
 // The whole resampler into the caller's CSR arrays `out` (offsets always; columns and values
 // only when nnz <= out.capacity), with one host synchronisation (a second only when the estimated

@@ -153,16 +153,19 @@ int max_long_dim(const fd_beam* beams, int n) {
 
 namespace {
 
-// Device descriptors of beamlets [i0, i1) with BEV volumes `stride` floats apart.  Beamlets of one
-// beam share the orientation, so the rotation matrices are computed once per run of equal angles.
-void build_descriptors(const fd_beam* beams, int i0, int i1, uint64_t stride, BeamletDev* out) {
+// Device descriptors of beamlets [i0, i1) with BEV volumes at offs[i] (packed), or `stride`
+// floats apart when offs is null.  Beamlets of one beam share the orientation, so the rotation
+// matrices are computed once per run of equal angles.
+void build_descriptors(const fd_beam* beams, int i0, int i1, uint64_t stride, const uint64_t* offs,
+                       BeamletDev* out) {
     for (int i = i0; i < i1; i++) {
+        const uint64_t off = offs ? offs[i] : (uint64_t)i * stride;
         if (i > i0 && std::memcmp(beams[i].angles, beams[i - 1].angles, sizeof(beams[i].angles)) == 0) {
             std::memcpy(out[i].Rf, out[i - 1].Rf, sizeof(out[i].Rf));
             std::memcpy(out[i].Ri, out[i - 1].Ri, sizeof(out[i].Ri));
-            fill_beamlet_dev_no_rotation(beams[i], (uint64_t)i * stride, out[i]);
+            fill_beamlet_dev_no_rotation(beams[i], off, out[i]);
         } else {
-            fill_beamlet_dev(beams[i], (uint64_t)i * stride, out[i]);
+            fill_beamlet_dev(beams[i], off, out[i]);
         }
     }
 }
@@ -177,10 +180,11 @@ void run_parts(int parts, const F& fn) {
     for (int p = 1; p < parts; p++) th[p].join();
 }
 
-// Pinned staging of the descriptors (beams non-null) and optionally the fluence of a batch; waits
-// until the previous upload from this workspace has left the staging area.
-int stage_inputs(Workspace& w, const fd_beam* beams, int n, uint64_t stride, const float* fluence_host,
-                 bool threads = false) {
+// Pinned staging of the descriptors (beams non-null; BEV offsets as build_descriptors) and
+// optionally the fluence of a batch; waits until the previous upload from this workspace has left
+// the staging area.
+int stage_inputs(Workspace& w, const fd_beam* beams, int n, uint64_t stride, const uint64_t* offs,
+                 const float* fluence_host, bool threads = false) {
     FD_HIP(hipEventSynchronize(w.stage_done));
     FD_TRY(grow_pinned(w.stage, w.stage_cap, (size_t)n));
     const size_t nf = fluence_host ? (size_t)n * kNpix : 0;
@@ -190,7 +194,7 @@ int stage_inputs(Workspace& w, const fd_beam* beams, int n, uint64_t stride, con
     const int parts = threads && n >= 2048 ? 8 : 1;
     run_parts(parts, [&](int p) {
         const int i0 = (int)((int64_t)n * p / parts), i1 = (int)((int64_t)n * (p + 1) / parts);
-        if (beams) build_descriptors(beams, i0, i1, stride, w.stage);
+        if (beams) build_descriptors(beams, i0, i1, stride, offs, w.stage);
         if (nf)
             std::memcpy(w.flu_stage + (size_t)i0 * kNpix, fluence_host + (size_t)i0 * kNpix,
                         (size_t)(i1 - i0) * kNpix * sizeof(float));
@@ -224,11 +228,14 @@ void update_candidate_estimate(Workspace& w, double ratio) {
 
 size_t batch_footprint(const fd_ctx* ctx, const fd_beam* beams, int n) {
     if (n <= 0) return 0;
-    const uint64_t stride = (uint64_t)max_long_dim(beams, n) * kNpix;
+    uint64_t bev = 0;  // BEV floats of the batch (packed volumes)
     double cand = 0;
-    for (int i = 0; i < n; i++) cand += frustum_voxels(beams[i], ctx->vol);
+    for (int i = 0; i < n; i++) {
+        bev += (uint64_t)beams[i].long_dim * kNpix;
+        cand += frustum_voxels(beams[i], ctx->vol);
+    }
     const size_t segs = (size_t)n * ctx->vol.dim[2];
-    return (size_t)n * stride * sizeof(float) * (3 + std::max(ctx->n_fwd, 4)) + (size_t)n * kNpix * sizeof(float) +
+    return (size_t)bev * sizeof(float) * (3 + std::max(ctx->n_fwd, 4)) + (size_t)n * kNpix * sizeof(float) +
            (size_t)(2.0 * cand) * (2 * sizeof(uint64_t) + 2 * sizeof(float)) + segs * 32 + (size_t)n * 128;
 }
 
@@ -238,14 +245,14 @@ int upload_beams(Workspace& w, const fd_beam* beams, int n, uint64_t stride, hip
     // device already holds these exact descriptors, ordered on this stream or already copied, the
     // upload (and the copy engine's hand-off between two kernels) is skipped.
     w.desc_next.resize((size_t)n);
-    build_descriptors(beams, 0, n, stride, w.desc_next.data());
+    build_descriptors(beams, 0, n, stride, nullptr, w.desc_next.data());
     const size_t bytes = (size_t)n * sizeof(BeamletDev);
     if (w.desc_valid && w.desc_host.size() == (size_t)n &&
         std::memcmp(w.desc_host.data(), w.desc_next.data(), bytes) == 0 &&
         (w.desc_stream == stream || hipEventQuery(w.stage_done) == hipSuccess))
         return FD_OK;
     w.desc_valid = false;
-    FD_TRY(stage_inputs(w, nullptr, n, stride, nullptr));
+    FD_TRY(stage_inputs(w, nullptr, n, stride, nullptr, nullptr));
     std::memcpy(w.stage, w.desc_next.data(), bytes);
     FD_TRY(w.desc.reserve(n));
     FD_HIP(hipMemcpyAsync(w.desc.ptr, w.stage, bytes, hipMemcpyHostToDevice, stream));
@@ -256,17 +263,17 @@ int upload_beams(Workspace& w, const fd_beam* beams, int n, uint64_t stride, hip
     return FD_OK;
 }
 
-int reserve_cccs(const fd_ctx* ctx, Workspace& w, int n, uint64_t stride);
+int reserve_cccs(const fd_ctx* ctx, Workspace& w, uint64_t bev_floats);
 
 int run_terma(const fd_ctx* ctx, Workspace& w, int options, int n, const float* fluence, float* terma, float* dens,
-              uint64_t stride, hipStream_t stream) {
+              uint64_t bev_floats, hipStream_t stream) {
     // FP64 radiological path on rays longer than kLongRayVoxels voxels (or on all rays)
     const float vmin = std::min(ctx->vol.voxel_size[0], std::min(ctx->vol.voxel_size[1], ctx->vol.voxel_size[2]));
     const float rpl64_len = (options & FD_OPT_RPL_FP64_ALL) ? -1.0f
                             : (options & FD_OPT_RPL_FP64) ? kLongRayVoxels * vmin
                                                           : INFINITY;
     // the per-cone partial buffer (4 floats per BEV value at least) is free until the CCCS stage
-    FD_TRY(reserve_cccs(ctx, w, n, stride));
+    FD_TRY(reserve_cccs(ctx, w, bev_floats));
     w.mark(FD_T_TERMA, 0, stream);
     hipError_t e = launch_terma(w.desc.ptr, n, fluence, ctx->density_pad, ctx->vol, ctx->spec_dev, rpl64_len, terma,
                                 dens, w.partial.ptr, ctx->tune.terma_segments, stream);
@@ -278,8 +285,10 @@ int run_terma(const fd_ctx* ctx, Workspace& w, int options, int n, const float* 
     return FD_OK;
 }
 
-int reserve_cccs(const fd_ctx* ctx, Workspace& w, int n, uint64_t stride) {
-    FD_TRY(w.partial.reserve((size_t)n * std::max(ctx->n_fwd, 4) * stride));  // also the Terma scratch
+// The per-cone partials of a batch whose BEV buffers hold bev_floats floats (n_fwd of them per BEV
+// value, and at least 4: the Terma kernel's scratch).
+int reserve_cccs(const fd_ctx* ctx, Workspace& w, uint64_t bev_floats) {
+    FD_TRY(w.partial.reserve((size_t)bev_floats * std::max(ctx->n_fwd, 4)));
     return FD_OK;
 }
 
@@ -295,8 +304,8 @@ int run_cccs_tables(const fd_ctx* ctx, Workspace& w, const CccsTables& tab, hipS
 }
 
 int run_cccs(const fd_ctx* ctx, Workspace& w, int options, int n, int max_L, const float* terma, const float* dens,
-             float* dose, uint64_t stride, hipStream_t stream, const CccsTables* tab, bool tab_built) {
-    FD_TRY(reserve_cccs(ctx, w, n, stride));
+             float* dose, uint64_t bev_floats, hipStream_t stream, const CccsTables* tab, bool tab_built) {
+    FD_TRY(reserve_cccs(ctx, w, bev_floats));
     if (tab && !tab_built) FD_TRY(run_cccs_tables(ctx, w, *tab, stream));
     hipEvent_t evs[3];
     const bool t = w.timing && w.ev[0];
@@ -308,7 +317,7 @@ int run_cccs(const fd_ctx* ctx, Workspace& w, int options, int n, int max_L, con
         evs[2] = w.ev[2 * FD_T_CONE_SUM + 1];
     }
     hipError_t e = launch_cccs(w.desc.ptr, n, max_L, ctx->cones_dev, ctx->n_fwd, ctx->cones.n_phi,
-                               (options & FD_OPT_EXACT_G) != 0, terma, dens, w.partial.ptr, dose, stride, stream,
+                               (options & FD_OPT_EXACT_G) != 0, terma, dens, w.partial.ptr, dose, stream,
                                t ? evs : nullptr, tab);
     if (t) {
         // the cone sum starts where the CCCS kernel stops (timings pair them)
@@ -558,18 +567,24 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
     out->nnz = 0;
     if (n == 0) return FD_OK;
     const int maxL = max_long_dim(beams, n);
-    const uint64_t stride = (uint64_t)maxL * kNpix;
+    // packed BEV volumes: beamlet i at bev_offs[i], the sum of L_j * 256 over j < i
+    w.bev_offs.resize((size_t)n + 1);
+    w.bev_offs[0] = 0;
     double est = 0;
-    for (int i = 0; i < n; i++) est += frustum_voxels(beams[i], ctx->vol);
+    for (int i = 0; i < n; i++) {
+        w.bev_offs[i + 1] = w.bev_offs[i] + (uint64_t)beams[i].long_dim * kNpix;
+        est += frustum_voxels(beams[i], ctx->vol);
+    }
+    const uint64_t bev = w.bev_offs[n];
     if (w.cand_per_beamlet <= 0) w.cand_per_beamlet = 2.0;  // candidates per frustum voxel, first guess
     uint64_t cap = std::max<uint64_t>((uint64_t)std::ceil(w.cand_per_beamlet * est), 1024);
 
     w.desc_valid = false;  // the batch's descriptors replace the single-stage ones
     FD_TRY(w.desc.reserve(n));
     FD_TRY(w.fluence.reserve((size_t)n * kNpix));
-    FD_TRY(w.terma.reserve((size_t)n * stride));
-    FD_TRY(w.dens_bev.reserve((size_t)n * stride));
-    FD_TRY(w.dose_bev.reserve((size_t)n * stride));
+    FD_TRY(w.terma.reserve((size_t)bev));
+    FD_TRY(w.dens_bev.reserve((size_t)bev));
+    FD_TRY(w.dose_bev.reserve((size_t)bev));
     FD_TRY(w.rowoff.reserve((size_t)n + 1));
     FD_TRY(reserve_segments(ctx, w, n));
     FD_TRY(w.tmp_cols.reserve(cap));
@@ -577,10 +592,10 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
     FD_TRY(w.out_cols.reserve(cap));
     FD_TRY(w.out_vals.reserve(cap));
 
-    FD_TRY(reserve_cccs(ctx, w, n, stride));
+    FD_TRY(reserve_cccs(ctx, w, bev));
     FD_TRY(reserve_host_rowoff(w, n));
     FD_TRY(plan_tables(ctx, w, options, beams, n));
-    FD_TRY(stage_inputs(w, beams, n, stride, fluence, overlap));
+    FD_TRY(stage_inputs(w, beams, n, 0, w.bev_offs.data(), fluence, overlap));
     const CccsTables tabs{w.tab_ent.ptr, w.tab_rec.ptr, w.tab_hdr.ptr, w.flags.ptr + 2, w.tab_n_gt};
     if (overlap && !w.aux_stream) {
         FD_HIP(hipStreamCreateWithFlags(&w.aux_stream, hipStreamNonBlocking));
@@ -618,9 +633,9 @@ int run_batch(const fd_ctx* ctx, Workspace& w, int options, const fd_beam* beams
             FD_TRY(launch_count(ctx, w, n, w.aux_stream));
             FD_HIP(hipEventRecord(w.count_ev, w.aux_stream));
         }
-        FD_TRY(run_terma(ctx, w, options, n, w.fluence.ptr, w.terma.ptr, w.dens_bev.ptr, stride, st));
+        FD_TRY(run_terma(ctx, w, options, n, w.fluence.ptr, w.terma.ptr, w.dens_bev.ptr, bev, st));
         if (overlap && w.tab_n_gt) FD_HIP(hipStreamWaitEvent(st, w.join_ev, 0));
-        FD_TRY(run_cccs(ctx, w, options, n, maxL, w.terma.ptr, w.dens_bev.ptr, w.dose_bev.ptr, stride, st,
+        FD_TRY(run_cccs(ctx, w, options, n, maxL, w.terma.ptr, w.dens_bev.ptr, w.dose_bev.ptr, bev, st,
                         w.tab_n_gt ? &tabs : nullptr, overlap));
         if (overlap)
             FD_HIP(hipStreamWaitEvent(st, w.count_ev, 0));

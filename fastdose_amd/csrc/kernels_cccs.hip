@@ -38,6 +38,13 @@ namespace {
 
 constexpr int kSlots = kRaysPerLane;  // voxels (rows) per lane
 
+// Per-cone partial dose of beamlet B for forward-cone slot t: the beamlet's n_fwd volumes of
+// L x 256 floats start at n_fwd x its BEV offset, so the partial buffer has the BEV layout of the
+// batch scaled by the cone count (n_fwd x the BEV floats of the batch).
+__device__ __forceinline__ float* cone_partial(float* partial, const BeamletDev& B, int n_fwd, int t) {
+    return partial + (uint64_t)B.bev_off * (uint64_t)n_fwd + (uint64_t)t * (uint64_t)B.L * kNpix;
+}
+
 // Distance to the next voxel boundary along one axis, in units of the direction's parameter
 // (d_nextPoint, DoseComp.cu:76-111): ceil(c + 1e-3) - c going up, floor(c - 1e-3) - c going
 // down, infinite for a zero direction component.  With sgn = +1 / -1 for d > 0 / otherwise this
@@ -336,8 +343,7 @@ template <bool EXACT_G, int APW>
 __device__ __forceinline__ void cccs_body(const BeamletDev* __restrict__ beams, int n,
                                           const ConeTable* __restrict__ cones, int n_fwd,
                                           const float* __restrict__ terma, const float* __restrict__ dens_bev,
-                                          float* __restrict__ partial, uint64_t stride, float* smem,
-                                          int* s_next) {
+                                          float* __restrict__ partial, float* smem, int* s_next) {
     const int tid = threadIdx.x;
     const int nthreads = blockDim.x;
     const int n_waves = nthreads >> 6;
@@ -404,7 +410,7 @@ __device__ __forceinline__ void cccs_body(const BeamletDev* __restrict__ beams, 
 
     const float* __restrict__ Tg = terma + B.bev_off;
     const float* __restrict__ Rg = dens_bev + B.bev_off;
-    float* __restrict__ P = partial + ((uint64_t)b * n_fwd + t) * stride;
+    float* __restrict__ P = cone_partial(partial, B, n_fwd, t);
     const bool two = nthreads < 2 * kNpix;    // each thread prefetches two slab values
     const bool prefetch = nthreads >= kNpix;  // smaller workgroups reload each slab
 
@@ -492,11 +498,10 @@ __device__ __forceinline__ void cccs_body(const BeamletDev* __restrict__ beams, 
 template <bool EXACT_G, int APW>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 8))) void cccs_inline_kernel(
     const BeamletDev* __restrict__ beams, int n, const ConeTable* __restrict__ cones, int n_fwd,
-    const float* __restrict__ terma, const float* __restrict__ dens_bev, float* __restrict__ partial,
-    uint64_t stride) {
+    const float* __restrict__ terma, const float* __restrict__ dens_bev, float* __restrict__ partial) {
     extern __shared__ float smem[];
     __shared__ int s_next[2];
-    cccs_body<EXACT_G, APW>(beams, n, cones, n_fwd, terma, dens_bev, partial, stride, smem, s_next);
+    cccs_body<EXACT_G, APW>(beams, n, cones, n_fwd, terma, dens_bev, partial, smem, s_next);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1290,8 +1295,7 @@ __device__ __forceinline__ void cccs_compute(const CcBlock& K, const CcLds& S, i
 template <bool EXACT_G>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(7, 8))) void cccs_kernel(
     const BeamletDev* __restrict__ beams, int n, const ConeTable* __restrict__ cones, int n_fwd,
-    const float* __restrict__ terma, const float* __restrict__ dens_bev, float* __restrict__ partial,
-    uint64_t stride) {
+    const float* __restrict__ terma, const float* __restrict__ dens_bev, float* __restrict__ partial) {
     extern __shared__ float4 lds4[];
     const int t = blockIdx.x / n;  // forward-cone slot, widest cone first
     const int b = blockIdx.x - t * n;
@@ -1304,7 +1308,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(7, 8))) vo
     K.n_phi = (blockDim.x >> 6) - 1;
     K.Tg = terma + B.bev_off;
     K.Rg = dens_bev + B.bev_off;
-    K.P = partial + ((uint64_t)b * n_fwd + t) * stride;
+    K.P = cone_partial(partial, B, n_fwd, t);
     K.two = K.nthreads < 2 * kNpix;  // each thread prefetches two slab values
     K.prefetch = K.nthreads >= kNpix;  // smaller workgroups reload each slab
     const CcLds S = carve_lds(lds4, K.n_phi);
@@ -1331,7 +1335,7 @@ template <bool EXACT_G>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8))) void cccs_tab_kernel(
     const BeamletDev* __restrict__ beams, int n, const ConeTable* __restrict__ cones, int n_fwd,
     const float* __restrict__ terma, const float* __restrict__ dens_bev, float* __restrict__ partial,
-    uint64_t stride, const TabEntry* __restrict__ tab, const float4* __restrict__ tab_rec,
+    const TabEntry* __restrict__ tab, const float4* __restrict__ tab_rec,
     const int* __restrict__ tab_hdr) {
     extern __shared__ float4 lds4[];
     const int t = blockIdx.x / n;  // forward-cone slot, widest cone first
@@ -1345,7 +1349,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8))) vo
     K.n_phi = blockDim.x >> 6;
     K.Tg = terma + B.bev_off;
     K.Rg = dens_bev + B.bev_off;
-    K.P = partial + ((uint64_t)b * n_fwd + t) * stride;
+    K.P = cone_partial(partial, B, n_fwd, t);
     K.two = K.nthreads < 2 * kNpix;
     K.prefetch = K.nthreads >= kNpix;
     const CcLds S = carve_lds_tab(lds4, K.n_phi);
@@ -1376,32 +1380,32 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6, 8))) vo
 // dose[b][s][p] = sum over forward cones of partial[b][t][s][p], cone slots in order.
 __global__ __launch_bounds__(256) void cone_sum_kernel(const BeamletDev* __restrict__ beams, int n_fwd,
                                                        const float* __restrict__ partial,
-                                                       float* __restrict__ dose, uint64_t stride) {
+                                                       float* __restrict__ dose) {
     const int b = blockIdx.y;
     const BeamletDev& B = beams[b];
     const uint64_t count = (uint64_t)B.L * kNpix;
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
-    const float* src = partial + (uint64_t)b * n_fwd * stride + i;
+    const float* src = partial + (uint64_t)B.bev_off * n_fwd + i;
     float acc = 0.0f;
-    for (int t = 0; t < n_fwd; t++) acc += src[(uint64_t)t * stride];
+    for (int t = 0; t < n_fwd; t++) acc += src[(uint64_t)t * count];
     dose[B.bev_off + i] = acc;
 }
 
-// The same, four values per thread, when the stride and the dose buffer are 16-byte aligned.
+// The same, four values per thread, when the partial and dose buffers are 16-byte aligned (every
+// beamlet's volumes start at multiples of 256 floats).
 __global__ __launch_bounds__(256) void cone_sum4_kernel(const BeamletDev* __restrict__ beams, int n_fwd,
                                                        const float* __restrict__ partial,
-                                                       float* __restrict__ dose, uint64_t stride) {
+                                                       float* __restrict__ dose) {
     const int b = blockIdx.y;
     const BeamletDev& B = beams[b];
     const uint64_t count4 = (uint64_t)B.L * (kNpix / 4);
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count4) return;
-    const float4* src = reinterpret_cast<const float4*>(partial + (uint64_t)b * n_fwd * stride) + i;
-    const uint64_t stride4 = stride / 4;
+    const float4* src = reinterpret_cast<const float4*>(partial + (uint64_t)B.bev_off * n_fwd) + i;
     float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     for (int t = 0; t < n_fwd; t++) {
-        const float4 v = src[(uint64_t)t * stride4];
+        const float4 v = src[(uint64_t)t * count4];
         acc.x += v.x;
         acc.y += v.y;
         acc.z += v.z;
@@ -1427,7 +1431,7 @@ hipError_t launch_cccs_tables(const CccsTables& tab, const ConeTable* cones_dev,
 
 hipError_t launch_cccs(const BeamletDev* beams, int n, int max_L, const ConeTable* cones_dev, int n_fwd,
                        int n_phi, bool exact_g, const float* terma, const float* dens_bev, float* partial,
-                       float* dose, uint64_t stride, hipStream_t stream, hipEvent_t* ev, const CccsTables* tab) {
+                       float* dose, hipStream_t stream, hipEvent_t* ev, const CccsTables* tab) {
     if (n <= 0) return hipSuccess;
     if (ev) (void)hipEventRecord(ev[0], stream);
     if (n_fwd > 0 && tab && n_phi <= kMaxMarchPhi) {
@@ -1436,10 +1440,10 @@ hipError_t launch_cccs(const BeamletDev* beams, int n, int max_L, const ConeTabl
         const size_t shm = tab_lds_bytes(n_phi);
         if (exact_g)
             hipLaunchKernelGGL(cccs_tab_kernel<true>, grid, block, shm, stream, beams, n, cones_dev, n_fwd, terma,
-                               dens_bev, partial, stride, tab->ent, tab->rec, tab->hdr);
+                               dens_bev, partial, tab->ent, tab->rec, tab->hdr);
         else
             hipLaunchKernelGGL(cccs_tab_kernel<false>, grid, block, shm, stream, beams, n, cones_dev, n_fwd, terma,
-                               dens_bev, partial, stride, tab->ent, tab->rec, tab->hdr);
+                               dens_bev, partial, tab->ent, tab->rec, tab->hdr);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     } else if (n_fwd > 0) {
@@ -1450,10 +1454,10 @@ hipError_t launch_cccs(const BeamletDev* beams, int n, int max_L, const ConeTabl
             const size_t shm = march_lds_bytes(n_phi);
             if (exact_g)
                 hipLaunchKernelGGL(cccs_kernel<true>, grid, block, shm, stream, beams, n, cones_dev, n_fwd, terma,
-                                   dens_bev, partial, stride);
+                                   dens_bev, partial);
             else
                 hipLaunchKernelGGL(cccs_kernel<false>, grid, block, shm, stream, beams, n, cones_dev, n_fwd, terma,
-                                   dens_bev, partial, stride);
+                                   dens_bev, partial);
         } else {
             // one azimuth per wavefront up to 16 azimuths (1024 threads), two beyond
             const int apw = n_phi <= kMaxPhiPerWave1 ? 1 : 2;
@@ -1463,30 +1467,29 @@ hipError_t launch_cccs(const BeamletDev* beams, int n, int max_L, const ConeTabl
             if (apw == 1) {
                 if (exact_g)
                     hipLaunchKernelGGL((cccs_inline_kernel<true, 1>), grid, block, shm, stream, beams, n, cones_dev,
-                                       n_fwd, terma, dens_bev, partial, stride);
+                                       n_fwd, terma, dens_bev, partial);
                 else
                     hipLaunchKernelGGL((cccs_inline_kernel<false, 1>), grid, block, shm, stream, beams, n, cones_dev,
-                                       n_fwd, terma, dens_bev, partial, stride);
+                                       n_fwd, terma, dens_bev, partial);
             } else {
                 if (exact_g)
                     hipLaunchKernelGGL((cccs_inline_kernel<true, 2>), grid, block, shm, stream, beams, n, cones_dev,
-                                       n_fwd, terma, dens_bev, partial, stride);
+                                       n_fwd, terma, dens_bev, partial);
                 else
                     hipLaunchKernelGGL((cccs_inline_kernel<false, 2>), grid, block, shm, stream, beams, n, cones_dev,
-                                       n_fwd, terma, dens_bev, partial, stride);
+                                       n_fwd, terma, dens_bev, partial);
             }
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     if (ev) (void)hipEventRecord(ev[1], stream);
-    if (stride % 4 == 0 && reinterpret_cast<uintptr_t>(dose) % 16 == 0 &&
-        reinterpret_cast<uintptr_t>(partial) % 16 == 0) {
+    if (reinterpret_cast<uintptr_t>(dose) % 16 == 0 && reinterpret_cast<uintptr_t>(partial) % 16 == 0) {
         dim3 g2((unsigned)(((uint64_t)max_L * (kNpix / 4) + 255) / 256), (unsigned)n), b2(256);
-        hipLaunchKernelGGL(cone_sum4_kernel, g2, b2, 0, stream, beams, n_fwd, partial, dose, stride);
+        hipLaunchKernelGGL(cone_sum4_kernel, g2, b2, 0, stream, beams, n_fwd, partial, dose);
     } else {
         dim3 g2((unsigned)(((uint64_t)max_L * kNpix + 255) / 256), (unsigned)n), b2(256);
-        hipLaunchKernelGGL(cone_sum_kernel, g2, b2, 0, stream, beams, n_fwd, partial, dose, stride);
+        hipLaunchKernelGGL(cone_sum_kernel, g2, b2, 0, stream, beams, n_fwd, partial, dose);
     }
     if (ev) (void)hipEventRecord(ev[2], stream);
     return hipGetLastError();
