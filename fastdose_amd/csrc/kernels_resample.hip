@@ -471,6 +471,10 @@ __global__ __launch_bounds__(256) void resample_compact_threshold_kernel(
     }
 }
 
+// Ordered copy of every segment's entries into the CSR arrays: a workgroup takes kCompactPlanes
+// planes of one beamlet, one wavefront per plane at a time (a segment holds a few hundred entries,
+// so a workgroup per segment would spend more on its launch than on its copy).
+constexpr int kCompactPlanes = 16;
 __global__ __launch_bounds__(256) void resample_compact_kernel(int Z, const uint64_t* __restrict__ cand_off,
                                                                const uint64_t* __restrict__ nnz_off,
                                                                const uint32_t* __restrict__ nnz_count,
@@ -481,12 +485,16 @@ __global__ __launch_bounds__(256) void resample_compact_kernel(int Z, const uint
     uint64_t* columns;
     float* values;
     if (!pick_dst(total[0], dst0, dst1, columns, values)) return;
-    const size_t seg = (size_t)blockIdx.y * Z + blockIdx.x;
-    const uint32_t cnt = nnz_count[seg];
-    const uint64_t src = cand_off[seg], dst = nnz_off[seg];
-    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
-        columns[dst + i] = tmp_cols[src + i];
-        values[dst + i] = tmp_vals[src + i];
+    const int z0 = (int)blockIdx.x * kCompactPlanes, z1 = min(z0 + kCompactPlanes, Z);
+    const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    for (int z = z0 + wave; z < z1; z += (int)(blockDim.x >> 6)) {
+        const size_t seg = (size_t)blockIdx.y * Z + z;
+        const uint32_t cnt = nnz_count[seg];
+        const uint64_t src = cand_off[seg], dst = nnz_off[seg];
+        for (uint32_t i = (uint32_t)lane; i < cnt; i += 64u) {
+            columns[dst + i] = tmp_cols[src + i];
+            values[dst + i] = tmp_vals[src + i];
+        }
     }
 }
 
@@ -565,8 +573,8 @@ hipError_t launch_resample_compact(int n, int Z, const uint64_t* cand_off, const
         hipLaunchKernelGGL(resample_compact_threshold_kernel, grid, block, 0, stream, Z, cand_off, nnz_off, nnz_count,
                            tmp_cols, tmp_vals, rowmax_bits, thr, total, dst0, dst1);
     else
-        hipLaunchKernelGGL(resample_compact_kernel, grid, block, 0, stream, Z, cand_off, nnz_off, nnz_count, tmp_cols,
-                           tmp_vals, total, dst0, dst1);
+        hipLaunchKernelGGL(resample_compact_kernel, dim3((Z + kCompactPlanes - 1) / kCompactPlanes, n), block, 0,
+                           stream, Z, cand_off, nnz_off, nnz_count, tmp_cols, tmp_vals, total, dst0, dst1);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // nnz_off has n*Z entries followed by the total at index n*Z
