@@ -471,10 +471,13 @@ __global__ __launch_bounds__(256) void resample_compact_threshold_kernel(
     }
 }
 
-// Ordered copy of every segment's entries into the CSR arrays: a workgroup takes kCompactPlanes
-// planes of one beamlet, one wavefront per plane at a time (a segment holds a few hundred entries,
-// so a workgroup per segment would spend more on its launch than on its copy).
+// Ordered copy of every segment's entries into the CSR arrays.  With PLANES > 1 a workgroup takes
+// that many planes of one beamlet, one wavefront per plane at a time (on large batches: a segment
+// holds a few hundred entries, so a workgroup per segment would spend more on its launch than on
+// its copy); with PLANES == 1 the workgroup copies one segment (small batches keep the parallelism).
 constexpr int kCompactPlanes = 16;
+constexpr uint64_t kCompactGroupSegments = 65536;  // segments of a batch from which planes are grouped
+template <int PLANES>
 __global__ __launch_bounds__(256) void resample_compact_kernel(int Z, const uint64_t* __restrict__ cand_off,
                                                                const uint64_t* __restrict__ nnz_off,
                                                                const uint32_t* __restrict__ nnz_count,
@@ -485,13 +488,16 @@ __global__ __launch_bounds__(256) void resample_compact_kernel(int Z, const uint
     uint64_t* columns;
     float* values;
     if (!pick_dst(total[0], dst0, dst1, columns, values)) return;
-    const int z0 = (int)blockIdx.x * kCompactPlanes, z1 = min(z0 + kCompactPlanes, Z);
-    const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
-    for (int z = z0 + wave; z < z1; z += (int)(blockDim.x >> 6)) {
+    constexpr bool kGrouped = PLANES > 1;
+    const int z0 = (int)blockIdx.x * PLANES, z1 = min(z0 + PLANES, Z);
+    const int first = kGrouped ? z0 + (int)(threadIdx.x >> 6) : z0;
+    const int zstep = kGrouped ? (int)(blockDim.x >> 6) : 1;
+    const uint32_t t0 = kGrouped ? (threadIdx.x & 63u) : threadIdx.x, tstep = kGrouped ? 64u : blockDim.x;
+    for (int z = first; z < z1; z += zstep) {
         const size_t seg = (size_t)blockIdx.y * Z + z;
         const uint32_t cnt = nnz_count[seg];
         const uint64_t src = cand_off[seg], dst = nnz_off[seg];
-        for (uint32_t i = (uint32_t)lane; i < cnt; i += 64u) {
+        for (uint32_t i = t0; i < cnt; i += tstep) {
             columns[dst + i] = tmp_cols[src + i];
             values[dst + i] = tmp_vals[src + i];
         }
@@ -569,12 +575,16 @@ hipError_t launch_resample_compact(int n, int Z, const uint64_t* cand_off, const
     const uint64_t* total = nnz_off + (size_t)n * Z;
     if (!dst0.columns || !dst0.values) dst0.capacity = 0;
     if (!dst1.columns || !dst1.values) dst1.capacity = 0;
-    if (rowmax_bits && thr > 0.0f)
+    if (rowmax_bits && thr > 0.0f) {
         hipLaunchKernelGGL(resample_compact_threshold_kernel, grid, block, 0, stream, Z, cand_off, nnz_off, nnz_count,
                            tmp_cols, tmp_vals, rowmax_bits, thr, total, dst0, dst1);
-    else
-        hipLaunchKernelGGL(resample_compact_kernel, dim3((Z + kCompactPlanes - 1) / kCompactPlanes, n), block, 0,
-                           stream, Z, cand_off, nnz_off, nnz_count, tmp_cols, tmp_vals, total, dst0, dst1);
+    } else if ((uint64_t)n * (uint64_t)Z >= kCompactGroupSegments) {
+        hipLaunchKernelGGL(resample_compact_kernel<kCompactPlanes>, dim3((Z + kCompactPlanes - 1) / kCompactPlanes, n),
+                           block, 0, stream, Z, cand_off, nnz_off, nnz_count, tmp_cols, tmp_vals, total, dst0, dst1);
+    } else {
+        hipLaunchKernelGGL(resample_compact_kernel<1>, grid, block, 0, stream, Z, cand_off, nnz_off, nnz_count,
+                           tmp_cols, tmp_vals, total, dst0, dst1);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // nnz_off has n*Z entries followed by the total at index n*Z
