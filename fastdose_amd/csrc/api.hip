@@ -298,7 +298,7 @@ static hipError_t rcp_exact_on_device(int device, bool& exact) {
         if (e == hipSuccess) e = hipMemset(d, 0, sizeof(uint32_t));
         if (e == hipSuccess) e = launch_rcp_check(-64, 128, d, nullptr);
         if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof(uint32_t), hipMemcpyDeviceToHost);
-        if (d) hipFree(d);
+        if (d) (void)hipFree(d);
         if (e != hipSuccess) return e;
         state[device] = h == 0 ? 1 : 2;
     }

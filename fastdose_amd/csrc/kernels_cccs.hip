@@ -1004,7 +1004,9 @@ struct TabRead {
 // LDS at lds + 16 lane (global_load_lds_dwordx4; lds wave-uniform).  Written as asm so that the
 // compiler, which does not see the LDS write, adds no wait for it before the wavefront's other LDS
 // reads; the caller waits (vmcnt) before the barrier after which the data is read.  The
-// instruction has no register result.
+// instruction has no register result.  m0 (the LDS address) is set in the same asm statement; no
+// instruction the compiler emits in these kernels reads m0 (no LDS DMA, s_movrel or message of its
+// own), so the clobber of a register the compiler reserves is safe here.
 __device__ __forceinline__ void load_to_lds16(const void* base, uint32_t off, void* lds) {
     const uint32_t a = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) void*)lds;
     asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(off), "s"(base), "s"(a) : "m0");

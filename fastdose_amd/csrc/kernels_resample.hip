@@ -177,7 +177,7 @@ __global__ __launch_bounds__(kRowThreads) void resample_count_kernel(const Beaml
         }
     }
     __syncthreads();
-    if (threadIdx.x < min(kCountPlanes, Z - z0)) {
+    if ((int)threadIdx.x < min(kCountPlanes, Z - z0)) {
         const size_t seg = (size_t)b * Z + z0 + threadIdx.x;
         const uint32_t cnt = s_cnt[threadIdx.x];
         cand_count[seg] = cnt;
