@@ -541,7 +541,7 @@ int fd_cccs(fd_ctx* ctx, const fd_beam* beams, int n, const float* terma_dev, co
     hipStream_t s = (hipStream_t)stream;
     FD_TRY(upload_beams(ctx->ws, beams, n, stride, s));
     return run_cccs(ctx, ctx->ws, ctx->options, n, max_long_dim(beams, n), terma_dev, density_bev_dev, dose_bev_dev,
-                    (uint64_t)n * stride, s);
+                    (uint64_t)n * stride, s, nullptr, false, stride % 4 == 0);
 }
 
 int fd_resample_csr(fd_ctx* ctx, const fd_beam* beams, int n, const float* dose_bev_dev, uint64_t stride,

@@ -263,7 +263,8 @@ int max_long_dim(const fd_beam* beams, int n);
 
 // Single stages on a workspace (descriptors uploaded by upload_beams first: BEV volumes `stride`
 // floats apart).  bev_floats: the floats each of the batch's BEV buffers spans (n x stride for the
-// single stages, the packed total in run_batch), which sizes the per-cone partials.
+// single stages, the packed total in run_batch), which sizes the per-cone partials.  vec4: every
+// beamlet's BEV offset is a multiple of four floats (the packed layout; a stride of 4k floats).
 int upload_beams(Workspace& w, const fd_beam* beams, int n, uint64_t stride, hipStream_t stream);
 int run_terma(const fd_ctx* ctx, Workspace& w, int options, int n, const float* fluence, float* terma, float* dens,
               uint64_t bev_floats, hipStream_t stream);
@@ -271,7 +272,7 @@ int run_terma(const fd_ctx* ctx, Workspace& w, int options, int n, const float* 
 int run_cccs_tables(const fd_ctx* ctx, Workspace& w, const CccsTables& tab, hipStream_t stream);
 int run_cccs(const fd_ctx* ctx, Workspace& w, int options, int n, int max_L, const float* terma, const float* dens,
              float* dose, uint64_t bev_floats, hipStream_t stream, const CccsTables* tab = nullptr,
-             bool tab_built = false);
+             bool tab_built = false, bool vec4 = true);
 // The whole resampler into the caller's CSR arrays `out` (offsets always; columns and values
 // only when nnz <= out.capacity), with one host synchronisation (a second only when the estimated
 // staging overflows).  thr > 0 keeps only entries >= thr * (the beamlet's maximum value).

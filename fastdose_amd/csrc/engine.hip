@@ -304,7 +304,8 @@ int run_cccs_tables(const fd_ctx* ctx, Workspace& w, const CccsTables& tab, hipS
 }
 
 int run_cccs(const fd_ctx* ctx, Workspace& w, int options, int n, int max_L, const float* terma, const float* dens,
-             float* dose, uint64_t bev_floats, hipStream_t stream, const CccsTables* tab, bool tab_built) {
+             float* dose, uint64_t bev_floats, hipStream_t stream, const CccsTables* tab, bool tab_built,
+             bool vec4) {
     FD_TRY(reserve_cccs(ctx, w, bev_floats));
     if (tab && !tab_built) FD_TRY(run_cccs_tables(ctx, w, *tab, stream));
     hipEvent_t evs[3];
@@ -317,7 +318,7 @@ int run_cccs(const fd_ctx* ctx, Workspace& w, int options, int n, int max_L, con
         evs[2] = w.ev[2 * FD_T_CONE_SUM + 1];
     }
     hipError_t e = launch_cccs(w.desc.ptr, n, max_L, ctx->cones_dev, ctx->n_fwd, ctx->cones.n_phi,
-                               (options & FD_OPT_EXACT_G) != 0, terma, dens, w.partial.ptr, dose, stream,
+                               (options & FD_OPT_EXACT_G) != 0, terma, dens, w.partial.ptr, dose, vec4, stream,
                                t ? evs : nullptr, tab);
     if (t) {
         // the cone sum starts where the CCCS kernel stops (timings pair them)

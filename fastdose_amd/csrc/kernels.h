@@ -52,14 +52,15 @@ int cccs_chunk_steps();  // lattice steps of one azimuth per chunk at most
 
 // CCCS over the forward cones (DoseComp.cu:217-392): per-cone partial BEV doses
 // partial[n_fwd * bev_off_b + t * L_b * 256 + s * 256 + p] (at least n_fwd times the batch's BEV
-// floats), then their sum into dose.
+// floats), then their sum into dose (four floats per thread when vec4: every bev_off_b a multiple
+// of four).
 // ev (nullable): events recorded before the CCCS kernel, between it and the cone sum, and after.
 // tab (nullable, n_phi <= cccs_table_max_phi()): the beamlets' march tables, built on this stream
 // beforehand (launch_cccs_tables), beams[b].geom indexing them; the CCCS reads its chunks from
 // them, with the results of the march wavefront.
 hipError_t launch_cccs(const BeamletDev* beams, int n, int max_L, const ConeTable* cones_dev, int n_fwd,
                        int n_phi, bool exact_g, const float* terma, const float* dens_bev, float* partial,
-                       float* dose, hipStream_t stream, hipEvent_t* ev = nullptr,
+                       float* dose, bool vec4, hipStream_t stream, hipEvent_t* ev = nullptr,
                        const CccsTables* tab = nullptr);
 
 // Resampling into CSR rows.  Stage 1 counts candidate voxels per (beamlet, z-plane) segment and

@@ -1394,8 +1394,8 @@ __global__ __launch_bounds__(256) void cone_sum_kernel(const BeamletDev* __restr
     dose[B.bev_off + i] = acc;
 }
 
-// The same, four values per thread, when the partial and dose buffers are 16-byte aligned (every
-// beamlet's volumes start at multiples of 256 floats).
+// The same, four values per thread, when the partial and dose buffers are 16-byte aligned and every
+// beamlet's BEV offset is a multiple of four floats.
 __global__ __launch_bounds__(256) void cone_sum4_kernel(const BeamletDev* __restrict__ beams, int n_fwd,
                                                        const float* __restrict__ partial,
                                                        float* __restrict__ dose) {
@@ -1433,7 +1433,7 @@ hipError_t launch_cccs_tables(const CccsTables& tab, const ConeTable* cones_dev,
 
 hipError_t launch_cccs(const BeamletDev* beams, int n, int max_L, const ConeTable* cones_dev, int n_fwd,
                        int n_phi, bool exact_g, const float* terma, const float* dens_bev, float* partial,
-                       float* dose, hipStream_t stream, hipEvent_t* ev, const CccsTables* tab) {
+                       float* dose, bool vec4, hipStream_t stream, hipEvent_t* ev, const CccsTables* tab) {
     if (n <= 0) return hipSuccess;
     if (ev) (void)hipEventRecord(ev[0], stream);
     if (n_fwd > 0 && tab && n_phi <= kMaxMarchPhi) {
@@ -1486,7 +1486,7 @@ hipError_t launch_cccs(const BeamletDev* beams, int n, int max_L, const ConeTabl
         if (e != hipSuccess) return e;
     }
     if (ev) (void)hipEventRecord(ev[1], stream);
-    if (reinterpret_cast<uintptr_t>(dose) % 16 == 0 && reinterpret_cast<uintptr_t>(partial) % 16 == 0) {
+    if (vec4 && reinterpret_cast<uintptr_t>(dose) % 16 == 0 && reinterpret_cast<uintptr_t>(partial) % 16 == 0) {
         dim3 g2((unsigned)(((uint64_t)max_L * (kNpix / 4) + 255) / 256), (unsigned)n), b2(256);
         hipLaunchKernelGGL(cone_sum4_kernel, g2, b2, 0, stream, beams, n_fwd, partial, dose);
     } else {
