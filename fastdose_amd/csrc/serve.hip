@@ -256,9 +256,16 @@ int fd_serve_start(fd_ctx* ctx, const fd_serve_opts* opts, fd_server** out) {
     if (o.extent == 0.0f) o.extent = 2.0f;
     FD_HIP(hipSetDevice(ctx->device));
     if (o.hbm_budget == 0) {
+        // half the free HBM, counting the scratch parked on the context (this server reuses it)
         size_t free_b = 0, total_b = 0;
         FD_HIP(hipMemGetInfo(&free_b, &total_b));
-        o.hbm_budget = free_b / 2;
+        size_t parked_b = 0;
+        {
+            std::lock_guard<std::mutex> lk(ctx->park_mu);
+            for (const Workspace& w : ctx->parked_ws) parked_b += w.footprint();
+            for (const CsrBufs& c : ctx->parked_csr) parked_b += c.cols.bytes() + c.vals.bytes();
+        }
+        o.hbm_budget = (free_b + parked_b) / 2;
     }
     fd_server* s = new fd_server();
     s->ctx = ctx;

@@ -356,7 +356,8 @@ typedef struct fd_server fd_server;
 typedef struct fd_serve_opts {
     int n_slots;                 /* concurrent batches (streams); 0: 3 */
     int max_batch_beamlets;      /* beamlets per batch; 0: the context's tuning (fd_autotune), else 4096 */
-    uint64_t hbm_budget;         /* bytes of batch scratch in flight; 0: half the free HBM */
+    uint64_t hbm_budget;         /* bytes of batch scratch in flight; 0: half the free HBM (with the
+                                    scratch of stopped servers the context keeps for reuse) */
     float extent;                /* resampler extent [cm]; 0: 2.0 */
     int options;                 /* FD_OPT_EXACT_G / FD_OPT_RPL_FP64(_ALL) for the batches */
     float threshold;             /* relative CSR threshold, as fd_ctx_set_threshold */
