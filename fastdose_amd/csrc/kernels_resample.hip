@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
+#include "xcd.h"
 
 namespace fdi {
 
@@ -319,7 +320,9 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(8, 
     __shared__ uint32_t s_wave[kRowThreads / 64 + 1];
     uint32_t* s_x0 = s_rows;
     uint32_t* s_off = s_rows + Y;
-    const int z = blockIdx.x, b = blockIdx.y;
+    // (z, b) from the XCD-grouped workgroup order: a beamlet's planes share one L2 for its BEV dose
+    const uint32_t wg = xcd_logical_id(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int z = (int)(wg % (uint32_t)Z), b = (int)(wg / (uint32_t)Z);
     const size_t seg = (size_t)b * Z + z;
     if (cand_off[seg + 1] == cand_off[seg]) {  // no candidate in this plane (most planes of a beamlet)
         if (threadIdx.x == 0) nnz_count[seg] = 0u;
